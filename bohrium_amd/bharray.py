@@ -1,0 +1,598 @@
+"""A lazy N-d array over the bhc bridge, in the style of the reference's bh107
+front end (bridge/bh107/bh107/bharray.py:49-572, ufuncs.py:257-400).
+
+Operations queue bytecode; data reaches the host only on request
+(`asnumpy()`), which syncs the base and flushes.  When the last Python
+reference to a base goes away a BH_FREE is queued, so temporaries created and
+freed inside one flush become registers in the fused kernel.
+"""
+import numbers
+
+import numpy as np
+
+from . import _bh_api as api
+from . import opcodes as ops
+
+
+class BhBase:
+    """Owner of one bridge base array; queues BH_FREE when collected."""
+
+    __slots__ = ("handle", "dtype", "nelem", "__weakref__")
+
+    def __init__(self, dtype, nelem):
+        self.dtype = np.dtype(dtype)
+        self.nelem = int(nelem)
+        self.handle = api.new(self.dtype, max(1, self.nelem))
+
+    def __del__(self):
+        try:
+            if self.handle is not None:
+                api.free(self.handle)
+        except Exception:
+            pass
+
+
+def _contiguous_strides(shape):
+    st, s = [], 1
+    for d in reversed(shape):
+        st.append(s)
+        s *= d
+    return tuple(reversed(st))
+
+
+class BhArray:
+    """A view (offset, shape, strides in elements) into a BhBase."""
+
+    __array_priority__ = 100
+
+    def __init__(self, shape, dtype=np.float64, base=None, offset=0, strides=None):
+        if isinstance(shape, numbers.Integral):
+            shape = (int(shape),)
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype) if base is None else base.dtype
+        n = 1
+        for s in self.shape:
+            n *= s
+        self.base = base if base is not None else BhBase(self.dtype, n)
+        self.offset = int(offset)
+        self.strides = tuple(strides) if strides is not None else _contiguous_strides(self.shape)
+
+    # ------------------------------------------------------------ basics
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    @property
+    def size(self):
+        n = 1
+        for s in self.shape:
+            n *= s
+        return n
+
+    def __len__(self):
+        return self.shape[0]
+
+    def iscontiguous(self):
+        return self.strides == _contiguous_strides(self.shape) or self.size <= 1
+
+    def empty(self):
+        return self.size == 0
+
+    def view_desc(self):
+        """(handle, start, shape, stride) of this view for the bridge; 0-d views are 1-element 1-d views."""
+        if self.ndim == 0:
+            return (self.base.handle, self.offset, (1,), (1,))
+        return (self.base.handle, self.offset, self.shape, self.strides)
+
+    def __repr__(self):
+        return "BhArray(%r, dtype=%s)" % (self.asnumpy(), self.dtype)
+
+    # ------------------------------------------------------------ data
+    def asnumpy(self):
+        """Flush and return a NumPy copy of this view."""
+        api.sync(self.base.handle)
+        api.flush()
+        host = api.data_copy_out(self.base.handle, self.dtype, max(1, self.base.nelem))
+        if self.size == 0:
+            return np.empty(self.shape, dtype=self.dtype)
+        it = host.itemsize
+        v = np.lib.stride_tricks.as_strided(host[self.offset:], shape=self.shape,
+                                            strides=tuple(s * it for s in self.strides))
+        return np.array(v, copy=True)
+
+    copy2numpy = asnumpy
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.asnumpy()
+        return a if dtype is None else a.astype(dtype)
+
+    def item(self):
+        return self.asnumpy().item()
+
+    def __float__(self):
+        return float(self.item())
+
+    def __int__(self):
+        return int(self.item())
+
+    def __bool__(self):
+        return bool(self.item())
+
+    # ------------------------------------------------------------ views
+    def _view(self, offset, shape, strides):
+        return BhArray(shape, base=self.base, offset=offset, strides=strides)
+
+    def __getitem__(self, key):
+        if not isinstance(key, tuple):
+            key = (key,)
+        if any(k is Ellipsis for k in key):
+            i = next(j for j, k in enumerate(key) if k is Ellipsis)
+            fill = self.ndim - (len(key) - 1 - sum(1 for k in key if k is None))
+            key = key[:i] + (slice(None),) * fill + key[i + 1:]
+        offset, shape, strides = self.offset, [], []
+        dim = 0
+        for k in key:
+            if k is None:
+                shape.append(1)
+                strides.append(0)
+                continue
+            if dim >= self.ndim:
+                raise IndexError("too many indices")
+            n, st = self.shape[dim], self.strides[dim]
+            if isinstance(k, slice):
+                start, stop, step = k.indices(n)
+                cnt = len(range(start, stop, step))
+                offset += start * st if cnt > 0 else 0
+                shape.append(cnt)
+                strides.append(st * step)
+            elif isinstance(k, numbers.Integral):
+                k = int(k)
+                if k < 0:
+                    k += n
+                if not 0 <= k < n:
+                    raise IndexError("index out of range")
+                offset += k * st
+            else:
+                raise TypeError("unsupported index %r" % (k,))
+            dim += 1
+        for d in range(dim, self.ndim):
+            shape.append(self.shape[d])
+            strides.append(self.strides[d])
+        return self._view(offset, tuple(shape), tuple(strides))
+
+    def __setitem__(self, key, value):
+        assign(value, self[key])
+
+    def reshape(self, *shape):
+        if len(shape) == 1 and isinstance(shape[0], (tuple, list)):
+            shape = tuple(shape[0])
+        shape = list(shape)
+        if -1 in shape:
+            i = shape.index(-1)
+            rest = 1
+            for j, s in enumerate(shape):
+                if j != i:
+                    rest *= s
+            shape[i] = self.size // max(1, rest)
+        shape = tuple(int(s) for s in shape)
+        src = self if self.iscontiguous() else self.copy()
+        return BhArray(shape, base=src.base, offset=src.offset, strides=_contiguous_strides(shape))
+
+    def flatten(self, always_copy=True):
+        if self.iscontiguous() and not always_copy:
+            return self.reshape(self.size)
+        return self.copy().reshape(self.size)
+
+    def ravel(self):
+        return self.flatten(always_copy=False)
+
+    def transpose(self, *axes):
+        if not axes:
+            axes = tuple(reversed(range(self.ndim)))
+        elif len(axes) == 1 and isinstance(axes[0], (tuple, list)):
+            axes = tuple(axes[0])
+        return self._view(self.offset, tuple(self.shape[a] for a in axes), tuple(self.strides[a] for a in axes))
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    def copy(self):
+        out = BhArray(self.shape, self.dtype)
+        ops_identity(self, out)
+        return out
+
+    def astype(self, dtype):
+        out = BhArray(self.shape, dtype)
+        ops_identity(self, out)
+        return out
+
+    def fill(self, value):
+        assign(value, self)
+
+    # ------------------------------------------------------------ reductions
+    def sum(self, axis=None):
+        return ufunc("add").reduce(self, axis)
+
+    def prod(self, axis=None):
+        return ufunc("multiply").reduce(self, axis)
+
+    def max(self, axis=None):
+        return ufunc("maximum").reduce(self, axis)
+
+    def min(self, axis=None):
+        return ufunc("minimum").reduce(self, axis)
+
+    def any(self, axis=None):
+        return ufunc("logical_or").reduce(self, axis)
+
+    def all(self, axis=None):
+        return ufunc("logical_and").reduce(self, axis)
+
+    def mean(self, axis=None):
+        s = self.sum(axis)
+        n = self.size if axis is None else self.shape[axis]
+        return s / n
+
+    def cumsum(self, axis=0):
+        return ufunc("add").accumulate(self, axis)
+
+    def cumprod(self, axis=0):
+        return ufunc("multiply").accumulate(self, axis)
+
+    # ------------------------------------------------------------ operators
+    def __add__(self, o): return ufunc("add")(self, o)
+    def __radd__(self, o): return ufunc("add")(o, self)
+    def __sub__(self, o): return ufunc("subtract")(self, o)
+    def __rsub__(self, o): return ufunc("subtract")(o, self)
+    def __mul__(self, o): return ufunc("multiply")(self, o)
+    def __rmul__(self, o): return ufunc("multiply")(o, self)
+    def __truediv__(self, o): return ufunc("true_divide")(self, o)
+    def __rtruediv__(self, o): return ufunc("true_divide")(o, self)
+    def __floordiv__(self, o): return ufunc("floor_divide")(self, o)
+    def __rfloordiv__(self, o): return ufunc("floor_divide")(o, self)
+    def __mod__(self, o): return ufunc("remainder")(self, o)
+    def __rmod__(self, o): return ufunc("remainder")(o, self)
+    def __pow__(self, o): return ufunc("power")(self, o)
+    def __rpow__(self, o): return ufunc("power")(o, self)
+    def __neg__(self): return ufunc("multiply")(self, -1)
+    def __pos__(self): return self
+    def __abs__(self): return ufunc("absolute")(self)
+    def __invert__(self): return ufunc("invert")(self)
+    def __and__(self, o): return ufunc("bitwise_and")(self, o)
+    def __or__(self, o): return ufunc("bitwise_or")(self, o)
+    def __xor__(self, o): return ufunc("bitwise_xor")(self, o)
+    def __lshift__(self, o): return ufunc("left_shift")(self, o)
+    def __rshift__(self, o): return ufunc("right_shift")(self, o)
+    def __lt__(self, o): return ufunc("less")(self, o)
+    def __le__(self, o): return ufunc("less_equal")(self, o)
+    def __gt__(self, o): return ufunc("greater")(self, o)
+    def __ge__(self, o): return ufunc("greater_equal")(self, o)
+    def __eq__(self, o): return ufunc("equal")(self, o)
+    def __ne__(self, o): return ufunc("not_equal")(self, o)
+    __hash__ = object.__hash__
+
+    def __iadd__(self, o): return ufunc("add")(self, o, out=self)
+    def __isub__(self, o): return ufunc("subtract")(self, o, out=self)
+    def __imul__(self, o): return ufunc("multiply")(self, o, out=self)
+    def __itruediv__(self, o): return ufunc("true_divide")(self, o, out=self)
+
+
+# ---------------------------------------------------------------- helpers
+
+def _broadcast_view(a, shape):
+    """View `a` broadcast to `shape` (zero strides)."""
+    if a.shape == tuple(shape):
+        return a
+    nd = len(shape)
+    pad = nd - a.ndim
+    if pad < 0:
+        raise ValueError("cannot broadcast %s to %s" % (a.shape, shape))
+    ash = (1,) * pad + a.shape
+    ast = (0,) * pad + a.strides
+    strides = []
+    for s_in, st, s_out in zip(ash, ast, shape):
+        if s_in == s_out:
+            strides.append(st)
+        elif s_in == 1:
+            strides.append(0)
+        else:
+            raise ValueError("cannot broadcast %s to %s" % (a.shape, shape))
+    return BhArray(tuple(shape), base=a.base, offset=a.offset, strides=tuple(strides))
+
+
+def _as_operand(x):
+    return x.view_desc() if isinstance(x, BhArray) else None
+
+
+def _same_memory(a, b):
+    return isinstance(a, BhArray) and isinstance(b, BhArray) and a.base is b.base
+
+
+def ops_identity(src, dst):
+    """dst[...] = src with a C cast (BH_IDENTITY)."""
+    if isinstance(src, BhArray):
+        api.op(ops.IDENTITY, [dst.view_desc(), _broadcast_view(src, dst.shape).view_desc()])
+    else:
+        api.op(ops.IDENTITY, [dst.view_desc(), None], const=_const_scalar(src, dst.dtype))
+
+
+def _const_scalar(value, dtype):
+    """A constant in the instruction's type, converted the way the reference's bridges do (C cast)."""
+    return np.asarray(value).astype(np.dtype(dtype))[()]
+
+
+def assign(value, out):
+    if isinstance(value, BhArray):
+        if value.dtype == out.dtype and value.shape == out.shape and value.base is out.base and \
+                value.offset == out.offset and value.strides == out.strides:
+            return out
+        if _same_memory(value, out):
+            value = value.copy()
+        ops_identity(value, out)
+    else:
+        ops_identity(value, out)
+    return out
+
+
+def array(obj, dtype=None):
+    """Create a BhArray holding a copy of `obj` (anything numpy.asarray accepts)."""
+    a = np.asarray(obj, dtype=dtype)
+    if a.dtype not in api.BH_TYPES:
+        a = a.astype(np.float64)
+    out = BhArray(a.shape, a.dtype)
+    if out.size:
+        api.data_copy_in(out.base.handle, np.ascontiguousarray(a).reshape(-1))
+    return out
+
+
+def empty(shape, dtype=np.float64):
+    return BhArray(shape, dtype)
+
+
+def full(shape, value, dtype=np.float64):
+    out = BhArray(shape, dtype)
+    if out.size:
+        ops_identity(value, out)
+    return out
+
+
+def zeros(shape, dtype=np.float64):
+    return full(shape, 0, dtype)
+
+
+def ones(shape, dtype=np.float64):
+    return full(shape, 1, dtype)
+
+
+def arange(n, dtype=np.int64):
+    out = BhArray((int(n),), dtype)
+    if n > 0:
+        api.op(ops.RANGE, [out.view_desc()])
+    return out
+
+
+def flush():
+    api.flush()
+
+
+def take(a, indices):
+    """out[i] = a.flat[indices[i]] for a contiguous `a` (BH_GATHER)."""
+    if not a.iscontiguous():
+        a = a.copy()
+    idx = indices if isinstance(indices, BhArray) else array(indices, np.int64)
+    if idx.dtype != np.int64:
+        idx = idx.astype(np.int64)
+    out = BhArray(idx.shape, a.dtype)
+    flat = BhArray((a.size,), base=a.base, offset=a.offset)
+    api.op(ops.GATHER, [out.view_desc(), flat.view_desc(), idx.view_desc()])
+    return out
+
+
+def put(a, indices, values, mask=None):
+    """a.flat[indices[i]] = values[i] (BH_SCATTER), only where mask[i] if given (BH_COND_SCATTER)."""
+    idx = indices if isinstance(indices, BhArray) else array(indices, np.int64)
+    if idx.dtype != np.int64:
+        idx = idx.astype(np.int64)
+    vals = values if isinstance(values, BhArray) else full(idx.shape, values, a.dtype)
+    if vals.dtype != a.dtype:
+        vals = vals.astype(a.dtype)
+    vals = _broadcast_view(vals, idx.shape)
+    flat = BhArray((a.size,), base=a.base, offset=a.offset)
+    if not a.iscontiguous():
+        raise ValueError("put() needs a contiguous target")
+    if mask is None:
+        api.op(ops.SCATTER, [flat.view_desc(), vals.view_desc(), idx.view_desc()])
+    else:
+        m = mask if isinstance(mask, BhArray) else array(mask, np.bool_)
+        api.op(ops.COND_SCATTER, [flat.view_desc(), vals.view_desc(), idx.view_desc(),
+                                  _broadcast_view(m, idx.shape).view_desc()])
+
+
+# ---------------------------------------------------------------- ufuncs
+
+_UFUNCS = {}
+
+
+def ufunc(name):
+    return _UFUNCS[name]
+
+
+class Ufunc:
+    """An elementwise operation mapped onto one opcode; dtypes are resolved as NumPy does."""
+
+    def __init__(self, name, opcode, nin, np_name=None, reduce_opcode=None, accumulate_opcode=None):
+        self.name = name
+        self.opcode = opcode
+        self.nin = nin
+        self.np_ufunc = getattr(np, np_name or name)
+        self.reduce_opcode = reduce_opcode
+        self.accumulate_opcode = accumulate_opcode
+        _UFUNCS[name] = self
+
+    def _loop_dtypes(self, args):
+        types = []
+        for a in args:
+            if isinstance(a, BhArray):
+                types.append(a.dtype)
+            elif isinstance(a, (bool, np.bool_)):
+                types.append(np.dtype(np.bool_))
+            elif isinstance(a, np.generic):
+                types.append(a.dtype)
+            elif isinstance(a, numbers.Integral):
+                types.append(int)
+            elif isinstance(a, numbers.Real):
+                types.append(float)
+            elif isinstance(a, numbers.Complex):
+                types.append(complex)
+            else:
+                types.append(np.asarray(a).dtype)
+        res = self.np_ufunc.resolve_dtypes(tuple(types) + (None,))
+        return res[:self.nin], res[self.nin]
+
+    def __call__(self, *args, out=None):
+        if len(args) != self.nin:
+            raise TypeError("%s takes %d inputs" % (self.name, self.nin))
+        in_types, out_type = self._loop_dtypes(args)
+        shapes = [a.shape for a in args if isinstance(a, BhArray)]
+        shape = np.broadcast_shapes(*shapes) if shapes else ()
+        if out is not None:
+            shape = out.shape
+        operands = []
+        const = None
+        for a, t in zip(args, in_types):
+            if isinstance(a, BhArray):
+                if a.dtype != t:
+                    a = a.astype(t)
+                operands.append(_broadcast_view(a, shape))
+            else:
+                if const is not None:
+                    a = full(shape, a, t)
+                    operands.append(a)
+                    continue
+                const = _const_scalar(a, t)
+                operands.append(None)
+        if const is not None and all(o is None for o in operands):
+            operands[0] = full(shape, args[0], in_types[0])
+            const = None if self.nin == 1 else const
+        res = out if (out is not None and out.dtype == out_type) else BhArray(shape, out_type)
+        self._emit(res, operands, const)
+        if out is not None and res is not out:
+            ops_identity(res, out)
+            return out
+        return res
+
+    def _emit(self, res, operands, const):
+        descs = [res.view_desc()] + [_as_operand(o) for o in operands]
+        if self.opcode is None:
+            raise NotImplementedError(self.name)
+        if callable(self.opcode):
+            self.opcode(res, operands, const)
+            return
+        api.op(self.opcode, descs, const=const)
+
+    # -------------------------------------------------------- reduce
+    def reduce(self, ary, axis=0, out=None):
+        if self.reduce_opcode is None:
+            raise NotImplementedError("%s.reduce" % self.name)
+        if not isinstance(ary, BhArray):
+            ary = array(ary)
+        if axis is None:
+            axis = list(range(ary.ndim))
+        elif isinstance(axis, numbers.Integral):
+            axis = [axis]
+        axis = sorted(a + ary.ndim if a < 0 else a for a in axis)
+        if not self.name.startswith("logical") and ary.dtype == np.bool_:
+            ary = ary.astype(np.uint64)  # counting True values, as the reference's bh107
+        if ary.ndim == 0:
+            return ary.copy()
+        if len(axis) > 1:
+            if len(axis) == ary.ndim and ary.iscontiguous():
+                return self.reduce(ary.reshape(ary.size), 0, out)
+            r = self.reduce(ary, axis[-1])
+            return self.reduce(r, axis[:-1], out)
+        ax = axis[0]
+        shape = tuple(s for i, s in enumerate(ary.shape) if i != ax)
+        tmp = BhArray(shape, ary.dtype)
+        if ary.shape[ax] == 0:
+            ident = {"add": 0, "multiply": 1, "logical_or": False, "logical_and": True}.get(self.name)
+            if ident is None:
+                raise ValueError("zero-size reduction without identity")
+            ops_identity(ident, tmp)
+        elif ary.shape[ax] == 1:
+            ops_identity(ary[(slice(None),) * ax + (0,)], tmp)
+        else:
+            api.op(self.reduce_opcode, [tmp.view_desc(), ary.view_desc(), None], const=np.int64(ax))
+        if out is not None:
+            assign(tmp, out)
+            return out
+        return tmp
+
+    def accumulate(self, ary, axis=0, out=None):
+        if self.accumulate_opcode is None:
+            raise NotImplementedError("%s.accumulate" % self.name)
+        if not isinstance(ary, BhArray):
+            ary = array(ary)
+        if axis < 0:
+            axis += ary.ndim
+        dt = np.dtype(np.int64) if ary.dtype.kind in "biu" and ary.dtype.itemsize < 8 else ary.dtype
+        if ary.dtype == np.uint64:
+            dt = np.dtype(np.uint64)
+        if dt != ary.dtype:
+            ary = ary.astype(dt)
+        res = BhArray(ary.shape, dt)
+        if ary.size:
+            api.op(self.accumulate_opcode, [res.view_desc(), ary.view_desc(), None], const=np.int64(axis))
+        if out is not None:
+            assign(res, out)
+            return out
+        return res
+
+
+def _floor_divide(res, operands, const):
+    """Integer: BH_DIVIDE (floor semantics for signed types); float: floor(a / b)."""
+    descs = [res.view_desc()] + [_as_operand(o) for o in operands]
+    api.op(ops.DIVIDE, descs, const=const)
+    if res.dtype.kind in "fc":
+        api.op(ops.FLOOR, [res.view_desc(), res.view_desc()])
+
+
+Ufunc("add", ops.ADD, 2, reduce_opcode=ops.ADD_REDUCE, accumulate_opcode=ops.ADD_ACCUMULATE)
+Ufunc("subtract", ops.SUBTRACT, 2)
+Ufunc("multiply", ops.MULTIPLY, 2, reduce_opcode=ops.MULTIPLY_REDUCE, accumulate_opcode=ops.MULTIPLY_ACCUMULATE)
+Ufunc("true_divide", ops.DIVIDE, 2, np_name="true_divide")
+Ufunc("divide", ops.DIVIDE, 2, np_name="true_divide")
+Ufunc("floor_divide", _floor_divide, 2)
+Ufunc("power", ops.POWER, 2)
+Ufunc("absolute", ops.ABSOLUTE, 1)
+Ufunc("greater", ops.GREATER, 2)
+Ufunc("greater_equal", ops.GREATER_EQUAL, 2)
+Ufunc("less", ops.LESS, 2)
+Ufunc("less_equal", ops.LESS_EQUAL, 2)
+Ufunc("equal", ops.EQUAL, 2)
+Ufunc("not_equal", ops.NOT_EQUAL, 2)
+Ufunc("logical_and", ops.LOGICAL_AND, 2, reduce_opcode=ops.LOGICAL_AND_REDUCE)
+Ufunc("logical_or", ops.LOGICAL_OR, 2, reduce_opcode=ops.LOGICAL_OR_REDUCE)
+Ufunc("logical_xor", ops.LOGICAL_XOR, 2, reduce_opcode=ops.LOGICAL_XOR_REDUCE)
+Ufunc("logical_not", ops.LOGICAL_NOT, 1)
+Ufunc("maximum", ops.MAXIMUM, 2, reduce_opcode=ops.MAXIMUM_REDUCE)
+Ufunc("minimum", ops.MINIMUM, 2, reduce_opcode=ops.MINIMUM_REDUCE)
+Ufunc("bitwise_and", ops.BITWISE_AND, 2, reduce_opcode=ops.BITWISE_AND_REDUCE)
+Ufunc("bitwise_or", ops.BITWISE_OR, 2, reduce_opcode=ops.BITWISE_OR_REDUCE)
+Ufunc("bitwise_xor", ops.BITWISE_XOR, 2, reduce_opcode=ops.BITWISE_XOR_REDUCE)
+Ufunc("invert", ops.INVERT, 1)
+Ufunc("left_shift", ops.LEFT_SHIFT, 2)
+Ufunc("right_shift", ops.RIGHT_SHIFT, 2)
+for _n, _o in [("cos", ops.COS), ("sin", ops.SIN), ("tan", ops.TAN), ("cosh", ops.COSH), ("sinh", ops.SINH),
+               ("tanh", ops.TANH), ("arcsin", ops.ARCSIN), ("arccos", ops.ARCCOS), ("arctan", ops.ARCTAN),
+               ("arcsinh", ops.ARCSINH), ("arccosh", ops.ARCCOSH), ("arctanh", ops.ARCTANH), ("exp", ops.EXP),
+               ("exp2", ops.EXP2), ("expm1", ops.EXPM1), ("log", ops.LOG), ("log2", ops.LOG2),
+               ("log10", ops.LOG10), ("log1p", ops.LOG1P), ("sqrt", ops.SQRT), ("ceil", ops.CEIL),
+               ("trunc", ops.TRUNC), ("floor", ops.FLOOR), ("rint", ops.RINT), ("isnan", ops.ISNAN),
+               ("isinf", ops.ISINF), ("isfinite", ops.ISFINITE), ("sign", ops.SIGN), ("real", ops.REAL),
+               ("imag", ops.IMAG), ("conjugate", ops.CONJ)]:
+    Ufunc(_n, _o, 1)
+Ufunc("arctan2", ops.ARCTAN2, 2)
+Ufunc("fmod", ops.MOD, 2)
+Ufunc("remainder", ops.REMAINDER, 2)
+Ufunc("mod", ops.REMAINDER, 2, np_name="remainder")

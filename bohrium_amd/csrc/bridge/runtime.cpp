@@ -1,0 +1,263 @@
+/* The bridge runtime behind the bhc C-ABI: queues instructions and flushes
+ * them as a BhIR into the first component of the stack (stack level 0).
+ * Queueing, the auto-flush threshold and BH_FREE ownership follow the
+ * reference bhxx runtime (bridge/cxx/src/Runtime.cpp:28-101). */
+#include "../../../include/bhc.h"
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <bohrium/bh_component.hpp>
+#include <bohrium/bh_main_memory.hpp>
+
+namespace {
+
+using bohrium::component::ComponentFace;
+
+struct Runtime {
+    std::unique_ptr<bohrium::ConfigParser> config;
+    ComponentFace stack;
+    std::vector<bh_instruction> queue;
+    std::set<bh_base *> syncs;
+    std::vector<bh_base *> freed;  // deleted after the flush that executes their BH_FREE
+    std::map<std::string, bh_opcode> extmethods;
+    bh_opcode next_ext = BH_MAX_OPCODE_ID + 1;
+    int64_t threshold = 1000;
+    std::string last_message;
+
+    explicit Runtime(const std::string &fallback) {
+        bohrium::ConfigParser::setFallbackPath(fallback);
+        config.reset(new bohrium::ConfigParser(-1));
+        const std::string top = config->getChildLibraryPath();
+        if (top.empty()) throw std::runtime_error("bhc: the configured stack is empty");
+        stack = ComponentFace{top, 0};
+    }
+
+    void enqueue(bh_instruction &&instr) {
+        queue.push_back(std::move(instr));
+        if (threshold > 0 && static_cast<int64_t>(queue.size()) >= threshold) flush(1, nullptr);
+    }
+
+    void flush(uint64_t nrepeats, bh_base *cond) {
+        BhIR bhir(std::move(queue), syncs, nrepeats, cond);
+        queue.clear();
+        syncs.clear();
+        stack.execute(&bhir);
+        for (bh_base *b : freed) delete b;
+        freed.clear();
+    }
+};
+
+Runtime *g_rt = nullptr;
+thread_local std::string g_err;
+
+Runtime &rt() {
+    if (g_rt == nullptr) throw std::runtime_error("bhc: not initialised (call bhc_init)");
+    return *g_rt;
+}
+
+bh_view to_view(const bhc_view_t &v) {
+    if (v.ndim < 0 || v.ndim > BHC_MAXDIM) throw std::invalid_argument("bhc: view ndim out of range");
+    bh_view ret;
+    ret.base = static_cast<bh_base *>(v.base);
+    ret.start = v.start;
+    ret.ndim = v.ndim;
+    ret.shape.assign(v.shape, v.shape + v.ndim);
+    ret.stride.assign(v.stride, v.stride + v.ndim);
+    return ret;
+}
+
+// Reject views that reach outside their base: the engine indexes device memory with them.
+void check_bounds(const bh_view &v) {
+    if (v.isConstant()) return;
+    int64_t lo = v.start, hi = v.start;
+    bool empty = false;
+    for (int64_t d = 0; d < v.ndim; ++d) {
+        if (v.shape[d] < 0) throw std::invalid_argument("bhc: negative shape");
+        if (v.shape[d] == 0) empty = true;
+        const int64_t ext = (v.shape[d] - 1) * v.stride[d];
+        if (ext < 0) lo += ext; else hi += ext;
+    }
+    if (!empty && (lo < 0 || hi >= v.base->nelem()))
+        throw std::out_of_range("bhc: view reaches outside its base array");
+}
+
+template<typename F>
+int guarded(F f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+bh_constant make_constant(int const_type, const void *value) {
+    bh_constant c;
+    c.type = static_cast<bh_type>(const_type);
+    if (const_type < 0 || const_type >= BH_NUM_TYPES) throw std::invalid_argument("bhc: bad constant type");
+    std::memcpy(&c.value, value, bh_type_size(c.type));
+    return c;
+}
+
+} // namespace
+
+extern "C" {
+
+int bhc_init(const char *fallback_config) {
+    return guarded([&] {
+        if (g_rt == nullptr) g_rt = new Runtime(fallback_config == nullptr ? "" : fallback_config);
+    });
+}
+
+int bhc_shutdown(void) {
+    return guarded([&] {
+        if (g_rt == nullptr) return;
+        g_rt->flush(1, nullptr);
+        delete g_rt;
+        g_rt = nullptr;
+    });
+}
+
+const char *bhc_last_error(void) { return g_err.c_str(); }
+
+bhc_base_t bhc_new(int dtype, int64_t nelem) {
+    bhc_base_t ret = nullptr;
+    guarded([&] {
+        if (dtype < 0 || dtype >= BH_NUM_TYPES) throw std::invalid_argument("bhc_new: bad dtype");
+        if (nelem < 0) throw std::invalid_argument("bhc_new: negative size");
+        ret = new bh_base(nelem, static_cast<bh_type>(dtype));
+    });
+    return ret;
+}
+
+int bhc_op(int64_t opcode, int nops, const bhc_view_t *views, int const_type, const void *const_value) {
+    return guarded([&] {
+        if (!bh_opcode_exists(opcode) || opcode == BH_FREE) throw std::invalid_argument("bhc_op: unknown opcode");
+        if (bh_noperands(opcode) != nops) throw std::invalid_argument("bhc_op: wrong number of operands");
+        bh_instruction instr;
+        instr.opcode = opcode;
+        bool has_const = false;
+        for (int i = 0; i < nops; ++i) {
+            instr.operand.push_back(to_view(views[i]));
+            if (instr.operand.back().isConstant()) {
+                has_const = true;
+            } else {
+                check_bounds(instr.operand.back());
+            }
+        }
+        if (has_const) instr.constant = make_constant(const_type, const_value);
+        if (nops > 0 && instr.operand[0].isConstant()) throw std::invalid_argument("bhc_op: output is a constant");
+        rt().enqueue(std::move(instr));
+    });
+}
+
+int bhc_random123(const bhc_view_t *out, uint64_t start, uint64_t key) {
+    return guarded([&] {
+        bh_instruction instr;
+        instr.opcode = BH_RANDOM;
+        instr.operand.push_back(to_view(*out));
+        check_bounds(instr.operand[0]);
+        instr.operand.emplace_back();
+        instr.constant.type = bh_type::R123;
+        instr.constant.value.r123 = {start, key};
+        rt().enqueue(std::move(instr));
+    });
+}
+
+int bhc_free(bhc_base_t base) {
+    return guarded([&] {
+        bh_base *b = static_cast<bh_base *>(base);
+        bh_instruction instr(BH_FREE, {bh_view(b)});
+        rt().freed.push_back(b);
+        rt().syncs.erase(b);
+        rt().enqueue(std::move(instr));
+    });
+}
+
+int bhc_sync(bhc_base_t base) {
+    return guarded([&] { rt().syncs.insert(static_cast<bh_base *>(base)); });
+}
+
+int bhc_flush(void) {
+    return guarded([&] { rt().flush(1, nullptr); });
+}
+
+int bhc_flush_and_repeat(uint64_t nrepeats, bhc_base_t condition) {
+    return guarded([&] { rt().flush(nrepeats, static_cast<bh_base *>(condition)); });
+}
+
+void *bhc_data_get(bhc_base_t base, int copy2host, int force_alloc, int nullify) {
+    void *ret = nullptr;
+    guarded([&] {
+        ret = rt().stack.getMemoryPointer(*static_cast<bh_base *>(base), copy2host != 0, force_alloc != 0, nullify != 0);
+    });
+    return ret;
+}
+
+int bhc_data_set(bhc_base_t base, int host_ptr, void *data) {
+    return guarded([&] { rt().stack.setMemoryPointer(static_cast<bh_base *>(base), host_ptr != 0, data); });
+}
+
+int bhc_data_copy_in(bhc_base_t base, const void *src, int64_t nbytes) {
+    return guarded([&] {
+        bh_base *b = static_cast<bh_base *>(base);
+        if (nbytes != b->nbytes()) throw std::invalid_argument("bhc_data_copy_in: size mismatch");
+        void *mem = bh_memory_malloc(static_cast<uint64_t>(nbytes));
+        std::memcpy(mem, src, static_cast<size_t>(nbytes));
+        rt().stack.setMemoryPointer(b, true, mem);
+    });
+}
+
+int bhc_data_copy_out(bhc_base_t base, void *dst, int64_t nbytes) {
+    return guarded([&] {
+        bh_base *b = static_cast<bh_base *>(base);
+        if (nbytes != b->nbytes()) throw std::invalid_argument("bhc_data_copy_out: size mismatch");
+        void *p = rt().stack.getMemoryPointer(*b, true, true, false);
+        std::memcpy(dst, p, static_cast<size_t>(nbytes));
+    });
+}
+
+const char *bhc_message(const char *msg) {
+    const char *ret = nullptr;
+    if (guarded([&] { rt().last_message = rt().stack.message(msg); }) == 0) ret = rt().last_message.c_str();
+    return ret;
+}
+
+int bhc_extmethod(const char *name, int nops, const bhc_view_t *views) {
+    return guarded([&] {
+        Runtime &r = rt();
+        auto it = r.extmethods.find(name);
+        bh_opcode opcode;
+        if (it == r.extmethods.end()) {
+            opcode = r.next_ext++;
+            r.stack.extmethod(name, opcode);
+            r.extmethods[name] = opcode;
+        } else {
+            opcode = it->second;
+        }
+        bh_instruction instr;
+        instr.opcode = opcode;
+        for (int i = 0; i < nops; ++i) {
+            instr.operand.push_back(to_view(views[i]));
+            check_bounds(instr.operand.back());
+        }
+        r.enqueue(std::move(instr));
+    });
+}
+
+int bhc_set_flush_threshold(int64_t n) {
+    return guarded([&] { rt().threshold = n; });
+}
+
+int64_t bhc_base_nelem(bhc_base_t base) { return static_cast<bh_base *>(base)->nelem(); }
+
+int bhc_base_dtype(bhc_base_t base) { return static_cast<int>(static_cast<bh_base *>(base)->dtype()); }
+
+} // extern "C"

@@ -1,0 +1,1030 @@
+#include "codegen.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace bohrium {
+namespace vehip {
+
+using jitk::Block;
+using jitk::InstrPtr;
+using jitk::LoopB;
+
+const char *strategy_name(Strategy s) {
+    switch (s) {
+        case Strategy::MAP: return "map";
+        case Strategy::FULL_REDUCE: return "reduce";
+        case Strategy::SCAN: return "scan";
+        case Strategy::SERIAL: return "serial";
+    }
+    return "?";
+}
+
+const char *hip_type(bh_type t) {
+    switch (t) {
+        case bh_type::BOOL: return "bool";
+        case bh_type::INT8: return "int8_t";
+        case bh_type::INT16: return "int16_t";
+        case bh_type::INT32: return "int32_t";
+        case bh_type::INT64: return "int64_t";
+        case bh_type::UINT8: return "uint8_t";
+        case bh_type::UINT16: return "uint16_t";
+        case bh_type::UINT32: return "uint32_t";
+        case bh_type::UINT64: return "uint64_t";
+        case bh_type::FLOAT32: return "float";
+        case bh_type::FLOAT64: return "double";
+        case bh_type::COMPLEX64: return "bh_c64";
+        case bh_type::COMPLEX128: return "bh_c128";
+        case bh_type::R123: return "uint64_t";
+    }
+    throw std::runtime_error("hip_type(): unknown type");
+}
+
+namespace {
+
+// ------------------------------------------------------------------ helpers
+
+bool sweeping_innermost_axis(const bh_instruction &i) {
+    return bh_opcode_is_sweep(i.opcode) && i.sweep_axis() == i.operand[1].ndim - 1;
+}
+
+bool is_complex_op(const bh_instruction &i, int idx) { return bh_type_is_complex(i.operand_type(idx)); }
+
+const char *cplx_prefix(bh_type t) { return t == bh_type::COMPLEX64 ? "make_c64" : "make_c128"; }
+
+// Conversion with C99 assignment semantics.
+std::string cast_expr(bh_type to, bh_type from, const std::string &e) {
+    const bool cto = bh_type_is_complex(to), cfrom = bh_type_is_complex(from);
+    if (!cto && !cfrom) return "((" + std::string(hip_type(to)) + ")(" + e + "))";
+    if (cto && !cfrom) {
+        const char *f = to == bh_type::COMPLEX64 ? "float" : "double";
+        return std::string(cplx_prefix(to)) + "((" + f + ")(" + e + "), (" + f + ")0)";
+    }
+    if (cto && cfrom) {
+        const char *f = to == bh_type::COMPLEX64 ? "float" : "double";
+        return std::string(cplx_prefix(to)) + "((" + f + ")(" + e + ").real, (" + f + ")(" + e + ").imag)";
+    }
+    if (to == bh_type::BOOL) return "((" + e + ").real != 0 || (" + e + ").imag != 0)";
+    return "((" + std::string(hip_type(to)) + ")(" + e + ").real)";
+}
+
+// Scalar body of one instruction; `ops` are the rendered operands in the
+// reference's order (out, in1, in2, ...).  Semantics follow the reference's
+// write_operation() for C99 (core/jitk/instruction.cpp:58-544).
+std::string hip_operation(const bh_instruction &instr, const std::vector<std::string> &ops) {
+    std::ostringstream o;
+    const auto &a = ops;
+    auto bin = [&](const char *op) { o << a[0] << " = " << a[1] << " " << op << " " << a[2] << ";"; };
+    auto un = [&](const char *fn, const char *cfn) {
+        if (bh_type_is_complex(instr.operand_type(1)) && cfn != nullptr)
+            o << a[0] << " = " << cfn << "(" << a[1] << ");";
+        else
+            o << a[0] << " = " << fn << "(" << a[1] << ");";
+    };
+    switch (instr.opcode) {
+        case BH_BITWISE_AND: bin("&"); break;
+        case BH_BITWISE_AND_REDUCE: o << a[0] << " = " << a[0] << " & " << a[1] << ";"; break;
+        case BH_BITWISE_OR: bin("|"); break;
+        case BH_BITWISE_OR_REDUCE: o << a[0] << " = " << a[0] << " | " << a[1] << ";"; break;
+        case BH_BITWISE_XOR: bin("^"); break;
+        case BH_BITWISE_XOR_REDUCE: o << a[0] << " = " << a[0] << " ^ " << a[1] << ";"; break;
+        case BH_LOGICAL_NOT: o << a[0] << " = !" << a[1] << ";"; break;
+        case BH_LOGICAL_OR: bin("||"); break;
+        case BH_LOGICAL_OR_REDUCE: o << a[0] << " = " << a[0] << " || " << a[1] << ";"; break;
+        case BH_LOGICAL_AND: bin("&&"); break;
+        case BH_LOGICAL_AND_REDUCE: o << a[0] << " = " << a[0] << " && " << a[1] << ";"; break;
+        case BH_LOGICAL_XOR: o << a[0] << " = !" << a[1] << " != !" << a[2] << ";"; break;
+        case BH_LOGICAL_XOR_REDUCE: o << a[0] << " = !" << a[0] << " != !" << a[1] << ";"; break;
+        case BH_LEFT_SHIFT: bin("<<"); break;
+        case BH_RIGHT_SHIFT: bin(">>"); break;
+        case BH_GREATER: bin(">"); break;
+        case BH_GREATER_EQUAL: bin(">="); break;
+        case BH_LESS: bin("<"); break;
+        case BH_LESS_EQUAL: bin("<="); break;
+        case BH_MAXIMUM: o << a[0] << " = " << a[1] << " > " << a[2] << " ? " << a[1] << " : " << a[2] << ";"; break;
+        case BH_MAXIMUM_REDUCE:
+            o << a[0] << " = " << a[0] << " > " << a[1] << " ? " << a[0] << " : " << a[1] << ";";
+            break;
+        case BH_MINIMUM: o << a[0] << " = " << a[1] << " < " << a[2] << " ? " << a[1] << " : " << a[2] << ";"; break;
+        case BH_MINIMUM_REDUCE:
+            o << a[0] << " = " << a[0] << " < " << a[1] << " ? " << a[0] << " : " << a[1] << ";";
+            break;
+        case BH_INVERT:
+            if (instr.operand[0].base->dtype() == bh_type::BOOL)
+                o << a[0] << " = !" << a[1] << ";";
+            else
+                o << a[0] << " = ~" << a[1] << ";";
+            break;
+        case BH_MOD:
+            if (bh_type_is_float(instr.operand[0].base->dtype()))
+                o << a[0] << " = fmod(" << a[1] << ", " << a[2] << ");";
+            else
+                o << a[0] << " = " << a[1] << " % " << a[2] << ";";
+            break;
+        case BH_REMAINDER: {
+            const bh_type t = instr.operand[0].base->dtype();
+            if (bh_type_is_float(t)) {
+                o << a[0] << " = " << a[1] << " - floor(" << a[1] << " / " << a[2] << ") * " << a[2] << ";";
+            } else if (bh_type_is_unsigned_integer(t)) {
+                o << a[0] << " = " << a[1] << " % " << a[2] << ";";
+            } else {  // Python/NumPy sign convention
+                o << a[0] << " = ((" << a[1] << " > 0) == (" << a[2] << " > 0) || (" << a[1] << " % " << a[2]
+                  << ") == 0) ? (" << a[1] << " % " << a[2] << ") : (" << a[1] << " % " << a[2] << ") + " << a[2]
+                  << ";";
+            }
+            break;
+        }
+        case BH_RINT: un("rint", nullptr); break;
+        case BH_EXP2: un("exp2", nullptr); break;
+        case BH_EXPM1: un("expm1", nullptr); break;
+        case BH_LOG1P: un("log1p", nullptr); break;
+        case BH_ARCSIN: un("asin", nullptr); break;
+        case BH_ARCCOS: un("acos", nullptr); break;
+        case BH_ARCTAN: un("atan", nullptr); break;
+        case BH_ARCTAN2: o << a[0] << " = atan2(" << a[1] << ", " << a[2] << ");"; break;
+        case BH_ARCSINH: un("asinh", nullptr); break;
+        case BH_ARCCOSH: un("acosh", nullptr); break;
+        case BH_ARCTANH: un("atanh", nullptr); break;
+        case BH_FLOOR: un("floor", nullptr); break;
+        case BH_CEIL: un("ceil", nullptr); break;
+        case BH_TRUNC: un("trunc", nullptr); break;
+        case BH_LOG2: un("log2", nullptr); break;
+        case BH_ISNAN:
+        case BH_ISINF:
+        case BH_ISFINITE: {
+            const char *fn = instr.opcode == BH_ISNAN ? "isnan" : (instr.opcode == BH_ISINF ? "isinf" : "isfinite");
+            const bh_type t = instr.operand_type(1);
+            if (bh_type_is_complex(t)) {
+                o << a[0] << " = " << fn << "((" << a[1] << ").real);";
+            } else if (bh_type_is_float(t)) {
+                o << a[0] << " = " << fn << "(" << a[1] << ");";
+            } else {
+                o << a[0] << " = " << (instr.opcode == BH_ISFINITE ? "true" : "false") << ";";
+            }
+            break;
+        }
+        case BH_CONJ: o << a[0] << " = bh_conj(" << a[1] << ");"; break;
+        case BH_RANGE:
+        case BH_RANDOM:
+        case BH_GATHER:
+        case BH_SCATTER: o << a[0] << " = " << a[1] << ";"; break;
+        case BH_COND_SCATTER: o << "if (" << a[2] << ") { " << a[0] << " = " << a[1] << "; }"; break;
+        case BH_SIN: un("sin", "bh_csin"); break;
+        case BH_COS: un("cos", "bh_ccos"); break;
+        case BH_TAN: un("tan", "bh_ctan"); break;
+        case BH_SINH: un("sinh", "bh_csinh"); break;
+        case BH_COSH: un("cosh", "bh_ccosh"); break;
+        case BH_TANH: un("tanh", "bh_ctanh"); break;
+        case BH_EXP: un("exp", "bh_cexp"); break;
+        case BH_ABSOLUTE: {
+            const bh_type t = instr.operand_type(1);
+            if (t == bh_type::BOOL || bh_type_is_unsigned_integer(t)) {
+                o << a[0] << " = " << a[1] << ";";
+            } else if (bh_type_is_float(t)) {
+                o << a[0] << " = fabs(" << a[1] << ");";
+            } else if (t == bh_type::INT64) {
+                o << a[0] << " = llabs(" << a[1] << ");";
+            } else if (bh_type_is_complex(t)) {
+                o << a[0] << " = abs((int)(" << a[1] << ").real);";  // C99 cast of a complex to int
+            } else {
+                o << a[0] << " = abs((int)" << a[1] << ");";
+            }
+            break;
+        }
+        case BH_SQRT: un("sqrt", "bh_csqrt"); break;
+        case BH_LOG: un("log", "bh_clog"); break;
+        case BH_NOT_EQUAL: bin("!="); break;
+        case BH_EQUAL: bin("=="); break;
+        case BH_POWER:
+            if (bh_type_is_complex(instr.operand_type(0))) {
+                o << a[0] << " = bh_cpow(" << a[1] << ", " << a[2] << ");";
+            } else if (bh_type_is_integer(instr.operand_type(0)) || instr.operand_type(0) == bh_type::BOOL) {
+                o << a[0] << " = pow((double) " << a[1] << ", (double)" << a[2] << ");";
+            } else {
+                o << a[0] << " = pow(" << a[1] << ", " << a[2] << ");";
+            }
+            break;
+        case BH_ADD: bin("+"); break;
+        case BH_ADD_REDUCE: o << a[0] << " += " << a[1] << ";"; break;
+        case BH_ADD_ACCUMULATE: bin("+"); break;
+        case BH_SUBTRACT: bin("-"); break;
+        case BH_MULTIPLY: bin("*"); break;
+        case BH_MULTIPLY_REDUCE: o << a[0] << " *= " << a[1] << ";"; break;
+        case BH_MULTIPLY_ACCUMULATE: bin("*"); break;
+        case BH_DIVIDE:
+            if (bh_type_is_signed_integer(instr.operand[0].base->dtype())) {  // NumPy floor division
+                o << a[0] << " = ((" << a[1] << " > 0) != (" << a[2] << " > 0) && (" << a[1] << " % " << a[2]
+                  << ") != 0)?(" << a[1] << " / " << a[2] << " - 1):(" << a[1] << " / " << a[2] << ");";
+            } else {
+                bin("/");
+            }
+            break;
+        case BH_IDENTITY:
+            o << a[0] << " = " << cast_expr(instr.operand_type(0), instr.operand_type(1), a[1]) << ";";
+            break;
+        case BH_LOG10:
+            if (bh_type_is_complex(instr.operand_type(0))) {
+                const bh_type t = instr.operand_type(0);
+                o << a[0] << " = bh_clog(" << a[1] << ") / " << cplx_prefix(t) << "(log(10.0f), 0);";
+            } else {
+                o << a[0] << " = log10(" << a[1] << ");";
+            }
+            break;
+        case BH_REAL: o << a[0] << " = bh_creal(" << a[1] << ");"; break;
+        case BH_IMAG: o << a[0] << " = bh_cimag(" << a[1] << ");"; break;
+        case BH_SIGN: {
+            const bh_type t = instr.operand_type(0);
+            if (bh_type_is_complex(t)) {
+                const char *ft = t == bh_type::COMPLEX64 ? "float" : "double";
+                o << "{ " << ft << " re_ = (" << a[1] << ").real, im_ = (" << a[1] << ").imag; " << a[0] << " = "
+                  << cplx_prefix(t) << "(re_ == 0 ? ((im_ > 0) - (0 > im_)) : ((re_ > 0) - (0 > re_)), 0); }";
+            } else {
+                o << a[0] << " = ((" << a[1] << " > 0) - (0 > " << a[1] << "));";
+            }
+            break;
+        }
+        default:
+            throw std::runtime_error(std::string("hip engine: instruction not supported: ") +
+                                     bh_opcode_text(instr.opcode));
+    }
+    return o.str();
+}
+
+// The binary operator of a reduction/accumulate as an expression of (x, y).
+std::string combine_expr(bh_opcode op, const std::string &x, const std::string &y) {
+    switch (op) {
+        case BH_ADD_REDUCE:
+        case BH_ADD_ACCUMULATE: return "(" + x + " + " + y + ")";
+        case BH_MULTIPLY_REDUCE:
+        case BH_MULTIPLY_ACCUMULATE: return "(" + x + " * " + y + ")";
+        case BH_MINIMUM_REDUCE: return "(" + x + " < " + y + " ? " + x + " : " + y + ")";
+        case BH_MAXIMUM_REDUCE: return "(" + x + " > " + y + " ? " + x + " : " + y + ")";
+        case BH_LOGICAL_AND_REDUCE: return "(" + x + " && " + y + ")";
+        case BH_LOGICAL_OR_REDUCE: return "(" + x + " || " + y + ")";
+        case BH_LOGICAL_XOR_REDUCE: return "(!" + x + " != !" + y + ")";
+        case BH_BITWISE_AND_REDUCE: return "(" + x + " & " + y + ")";
+        case BH_BITWISE_OR_REDUCE: return "(" + x + " | " + y + ")";
+        case BH_BITWISE_XOR_REDUCE: return "(" + x + " ^ " + y + ")";
+        default: throw std::runtime_error("combine_expr(): not a sweep");
+    }
+}
+
+std::string literal(const bh_constant &c) {
+    std::ostringstream s;
+    c.write_literal(s);
+    return s.str();
+}
+
+// A view key that ignores size-1 dimensions (which never change an address).
+struct VKey {
+    const bh_base *base;
+    int64_t start;
+    std::vector<std::pair<int64_t, int64_t> > dims;
+    bool operator<(const VKey &o) const {
+        if (base != o.base) return base < o.base;
+        if (start != o.start) return start < o.start;
+        return dims < o.dims;
+    }
+};
+VKey vkey(const bh_view &v) {
+    VKey k{v.base, v.start, {}};
+    for (int64_t d = 0; d < v.ndim; ++d)
+        if (v.shape[d] > 1) k.dims.emplace_back(v.shape[d], v.stride[d]);
+    return k;
+}
+
+// Key of an offset-and-strides argument group: views with equal start and strides share it.
+typedef std::pair<int64_t, std::vector<int64_t> > OSKey;
+OSKey oskey(const bh_view &v) { return {v.start, std::vector<int64_t>(v.stride.begin(), v.stride.end())}; }
+
+struct Scope {
+    const Scope *parent = nullptr;
+    std::set<const bh_base *> tmps;
+    std::set<VKey> srep;
+    bool isTmp(const bh_base *b) const { return tmps.count(b) || (parent && parent->isTmp(b)); }
+    bool isSrep(const bh_view &v) const { return srep.count(vkey(v)) || (parent && parent->isSrep(v)); }
+    bool isArray(const bh_view &v) const { return !(isTmp(v.base) || isSrep(v)); }
+    bool isDeclared(const bh_view &v) const { return isTmp(v.base) || isSrep(v); }
+};
+
+void indent(std::ostringstream &o, int n) { o << std::string(n * 2, ' '); }
+
+// ------------------------------------------------------------------ writer
+
+class Writer {
+public:
+    Writer(const LoopB &unit, const WriterConfig &cfg) : unit(unit), cfg(cfg) { collect(); }
+
+    KernelPlan plan(Strategy s);
+
+private:
+    const LoopB &unit;
+    const WriterConfig &cfg;
+    KernelPlan kp;
+
+    std::map<const bh_base *, int> bid;
+    std::vector<const bh_base *> base_order;
+    std::set<const bh_base *> always_array;
+    std::set<bh_base *> temps;
+    std::map<VKey, int> vid;
+    std::map<OSKey, int> osid;
+    std::vector<const bh_view *> os_views;
+    std::map<const LoopB *, int> lid;
+    std::vector<const LoopB *> loops;
+    std::vector<InstrPtr> const_instrs;
+    std::map<int64_t, int> const_of_origin;
+    std::set<const bh_base *> index_checked;  // bases needing their element count (gather/scatter)
+    // Instruction replaced by "store its input into `scan_sink`" in the scan skeleton.
+    const bh_instruction *scan_instr = nullptr;
+    std::string scan_sink;
+
+    std::ostringstream body;
+
+    void collect();
+    int view_id(const bh_view &v) {
+        auto it = vid.find(vkey(v));
+        if (it != vid.end()) return it->second;
+        const int id = static_cast<int>(vid.size());
+        vid[vkey(v)] = id;
+        return id;
+    }
+    int os_id(const bh_view &v) {
+        auto it = osid.find(oskey(v));
+        if (it != osid.end()) return it->second;
+        const int id = static_cast<int>(os_views.size());
+        osid[oskey(v)] = id;
+        os_views.push_back(&v);
+        return id;
+    }
+    std::string name(const Scope &sc, const bh_view &v) {
+        std::ostringstream s;
+        if (sc.isTmp(v.base)) {
+            s << "t" << bid.at(v.base);
+        } else if (sc.isSrep(v)) {
+            s << "s" << bid.at(v.base) << "_" << view_id(v);
+        } else {
+            s << "a" << bid.at(v.base);
+        }
+        return s.str();
+    }
+    std::string index(const bh_view &v, int hidden_axis = BH_MAXDIM, int off_axis = -1, int off = 0) {
+        std::ostringstream s;
+        const int k = os_id(v);
+        s << "vo" << k;
+        if (!v.is_scalar()) {
+            for (int64_t d = 0; d < v.ndim; ++d) {
+                int64_t t = d >= hidden_axis ? d + 1 : d;
+                if (t == off_axis)
+                    s << " + (i" << t << " + (i" << t << " == 0 ? 0 : " << off << "))";
+                else
+                    s << " + i" << t;
+                s << " * vs" << k << "_" << d;
+            }
+        }
+        return s.str();
+    }
+    std::string ref(const Scope &sc, const bh_view &v, int hidden_axis = BH_MAXDIM) {
+        std::string n = name(sc, v);
+        if (sc.isArray(v)) n += "[" + index(v, hidden_axis) + "]";
+        return n;
+    }
+    std::string const_ref(const bh_instruction &i) { return "c" + std::to_string(const_of_origin.at(i.origin_id)); }
+
+    void declare_scope(const LoopB &l, Scope &sc, int ind);
+    void write_back(const LoopB &l, Scope &sc, int ind);
+    void emit_blocks(const LoopB &l, Scope &sc, int ind);
+    void emit_body(const LoopB &l, const Scope *parent, int ind);
+    void emit_instr(const bh_instruction &i, Scope &sc, int ind);
+
+    void emit_map();
+    void emit_full_reduce();
+    void emit_scan();
+    void emit_serial();
+    std::string signature(const std::string &extra);
+};
+
+void Writer::collect() {
+    std::vector<InstrPtr> all = jitk::allInstr(unit);
+    for (const InstrPtr &i : all) {
+        for (const bh_view &v : i->operand) {
+            if (v.isConstant()) continue;
+            if (!bid.count(v.base)) {
+                bid[v.base] = static_cast<int>(bid.size());
+                base_order.push_back(v.base);
+            }
+        }
+        if (bh_opcode_is_accumulate(i->opcode)) {
+            always_array.insert(i->operand[0].base);
+        } else if (i->opcode == BH_GATHER) {
+            always_array.insert(i->operand[1].base);
+            index_checked.insert(i->operand[1].base);
+        } else if (i->opcode == BH_SCATTER || i->opcode == BH_COND_SCATTER) {
+            always_array.insert(i->operand[0].base);
+            index_checked.insert(i->operand[0].base);
+        }
+        if (i->has_constant()) const_instrs.push_back(i);
+    }
+    for (const bh_base *b : unit.getAllFrees()) {
+        if (!bid.count(b)) {
+            bid[b] = static_cast<int>(bid.size());
+            base_order.push_back(b);
+        }
+    }
+    std::sort(const_instrs.begin(), const_instrs.end(),
+              [](const InstrPtr &a, const InstrPtr &b) { return a->origin_id < b->origin_id; });
+    for (size_t j = 0; j < const_instrs.size(); ++j) const_of_origin[const_instrs[j]->origin_id] = static_cast<int>(j);
+    temps = unit.getAllTemps();
+    std::set<bh_base *> non_temps = unit.getAllNonTemps();
+    for (const bh_base *b : always_array) non_temps.insert(const_cast<bh_base *>(b));
+    for (const InstrPtr &i : all)
+        for (const bh_view &v : i->operand)
+            if (!v.isConstant() && non_temps.count(v.base) &&
+                std::find(kp.params.begin(), kp.params.end(), v.base) == kp.params.end())
+                kp.params.push_back(v.base);
+    for (const InstrPtr &i : all) {
+        if (i->operand.empty() || i->operand[0].isConstant()) continue;
+        if (non_temps.count(i->operand[0].base)) kp.written.insert(i->operand[0].base);
+    }
+    // Loop ids (pre-order) for the size arguments.
+    std::vector<const LoopB *> stack{&unit};
+    while (!stack.empty()) {
+        const LoopB *l = stack.back();
+        stack.pop_back();
+        if (l->rank >= 0 && !lid.count(l)) {
+            lid[l] = static_cast<int>(loops.size());
+            loops.push_back(l);
+        }
+        for (auto it = l->_block_list.rbegin(); it != l->_block_list.rend(); ++it)
+            if (!it->isInstr()) stack.push_back(&it->getLoop());
+    }
+    // Register every view's offset-and-stride group in instruction order.
+    for (const InstrPtr &i : all)
+        for (const bh_view &v : i->operand)
+            if (!v.isConstant()) os_id(v);
+}
+
+void Writer::declare_scope(const LoopB &l, Scope &sc, int ind) {
+    const std::set<bh_base *> local_tmps = l.getLocalTemps();
+    for (const InstrPtr &i : jitk::allInstr(l)) {
+        for (const bh_view &v : i->operand) {
+            if (v.isConstant() || !local_tmps.count(v.base)) continue;
+            if (sc.isDeclared(v) || always_array.count(v.base)) continue;
+            sc.tmps.insert(v.base);
+            indent(body, ind);
+            body << hip_type(v.base->dtype()) << " t" << bid.at(v.base) << ";\n";
+        }
+    }
+    for (const Block &b : l._block_list) {
+        if (b.isInstr()) continue;
+        for (const InstrPtr &i : jitk::allLocalInstr(b.getLoop())) {
+            if (!(bh_opcode_is_reduction(i->opcode) && sweeping_innermost_axis(*i))) continue;
+            const bh_view &v = i->operand[0];
+            if (sc.isDeclared(v) || always_array.count(v.base)) continue;
+            sc.srep.insert(vkey(v));
+            indent(body, ind);
+            body << hip_type(v.base->dtype()) << " s" << bid.at(v.base) << "_" << view_id(v) << ";\n";
+        }
+    }
+}
+
+void Writer::write_back(const LoopB &l, Scope &sc, int ind) {
+    for (const InstrPtr &i : jitk::allLocalInstr(l)) {
+        if (i->operand.empty() || i->operand[0].isConstant()) continue;
+        const bh_view &v = i->operand[0];
+        if (!sc.isSrep(v)) continue;
+        indent(body, ind);
+        body << "a" << bid.at(v.base) << "["
+             << index(v, bh_opcode_is_reduction(i->opcode) ? i->sweep_axis() : BH_MAXDIM) << "] = s"
+             << bid.at(v.base) << "_" << view_id(v) << ";\n";
+        sc.srep.erase(vkey(v));
+    }
+}
+
+void Writer::emit_blocks(const LoopB &l, Scope &sc, int ind) {
+    for (const Block &b : l._block_list) {
+        if (b.isInstr()) {
+            if (b.getInstr() != nullptr && !bh_opcode_is_system(b.getInstr()->opcode))
+                emit_instr(*b.getInstr(), sc, ind);
+        } else {
+            const LoopB &c = b.getLoop();
+            indent(body, ind);
+            body << "for (int64_t i" << c.rank << " = 0; i" << c.rank << " < n" << lid.at(&c) << "; ++i" << c.rank
+                 << ") {\n";
+            emit_body(c, &sc, ind + 1);
+            indent(body, ind);
+            body << "}\n";
+        }
+    }
+}
+
+void Writer::emit_body(const LoopB &l, const Scope *parent, int ind) {
+    if (l.isSystemOnly()) return;
+    Scope sc;
+    sc.parent = parent;
+    declare_scope(l, sc, ind);
+    emit_blocks(l, sc, ind);
+    write_back(l, sc, ind);
+}
+
+void Writer::emit_instr(const bh_instruction &instr, Scope &sc, int ind) {
+    std::vector<std::string> ops;
+    indent(body, ind);
+    if (&instr == scan_instr) {
+        body << scan_sink << " = " << ref(sc, instr.operand[1]) << ";\n";
+        return;
+    }
+    if (instr.opcode == BH_RANGE) {
+        ops.push_back(ref(sc, instr.operand[0]));
+        ops.push_back("(" + index(instr.operand[0]) + ")");
+    } else if (instr.opcode == BH_RANDOM) {
+        ops.push_back(ref(sc, instr.operand[0]));
+        const std::string c = const_ref(instr);
+        ops.push_back("bh_random123(" + c + "_start, " + c + "_key, " + index(instr.operand[0]) + ")");
+    } else if (instr.opcode == BH_GATHER) {
+        // out[...] = in[in.start + idx[...]], with the index checked against the base.
+        const bh_view &in = instr.operand[1];
+        const int k = os_id(in);
+        const std::string out = ref(sc, instr.operand[0]);
+        body << "{ const int64_t g_ = vo" << k << " + (int64_t)(" << ref(sc, instr.operand[2]) << "); "
+             << "if (g_ < 0 || g_ >= nb" << bid.at(in.base) << ") { atomicOr(bh_err, BH_ERR_INDEX); } else { "
+             << out << " = a" << bid.at(in.base) << "[g_]; } }\n";
+        kp.uses_index_check = true;
+        return;
+    } else if (instr.opcode == BH_SCATTER || instr.opcode == BH_COND_SCATTER) {
+        const bh_view &out = instr.operand[0];
+        const int k = os_id(out);
+        body << "{ const int64_t g_ = vo" << k << " + (int64_t)(" << ref(sc, instr.operand[2]) << "); ";
+        if (instr.opcode == BH_COND_SCATTER) body << "if (" << ref(sc, instr.operand[3]) << ") ";
+        body << "{ if (g_ < 0 || g_ >= nb" << bid.at(out.base) << ") { atomicOr(bh_err, BH_ERR_INDEX); } else { a"
+             << bid.at(out.base) << "[g_] = " << ref(sc, instr.operand[1]) << "; } } }\n";
+        kp.uses_index_check = true;
+        return;
+    } else if (bh_opcode_is_accumulate(instr.opcode)) {
+        ops.push_back(ref(sc, instr.operand[0]));
+        ops.push_back(name(sc, instr.operand[0]) + "[" +
+                      index(instr.operand[0], BH_MAXDIM, instr.sweep_axis(), -1) + "]");
+        ops.push_back(ref(sc, instr.operand[1]));
+    } else {
+        for (size_t o = 0; o < instr.operand.size(); ++o) {
+            const bh_view &v = instr.operand[o];
+            if (v.isConstant()) {
+                ops.push_back(const_ref(instr));
+            } else if (o == 0 && bh_opcode_is_reduction(instr.opcode) && instr.operand[1].ndim > 1) {
+                ops.push_back(ref(sc, v, instr.sweep_axis()));
+            } else {
+                ops.push_back(ref(sc, v));
+            }
+        }
+    }
+    body << hip_operation(instr, ops) << "\n";
+}
+
+std::string Writer::signature(const std::string &extra) {
+    std::ostringstream s;
+    std::vector<std::string> params;
+    auto lit = [&](const std::string &decl, const void *p, int size, int align) {
+        params.push_back(decl);
+        ArgSpec a;
+        a.kind = ArgSpec::LITERAL;
+        a.bytes.assign(static_cast<const uint8_t *>(p), static_cast<const uint8_t *>(p) + size);
+        a.size = size;
+        a.align = align;
+        kp.args.push_back(a);
+    };
+    for (bh_base *b : kp.params) {
+        params.push_back(std::string(hip_type(b->dtype())) + " *__restrict__ a" + std::to_string(bid.at(b)));
+        ArgSpec a;
+        a.kind = ArgSpec::BASE_PTR;
+        a.base = b;
+        kp.args.push_back(a);
+    }
+    for (size_t k = 0; k < os_views.size(); ++k) {
+        const bh_view &v = *os_views[k];
+        lit("const int64_t vo" + std::to_string(k), &v.start, 8, 8);
+        for (int64_t d = 0; d < v.ndim; ++d)
+            lit("const int64_t vs" + std::to_string(k) + "_" + std::to_string(d), &v.stride[d], 8, 8);
+    }
+    for (size_t j = 0; j < loops.size(); ++j) lit("const int64_t n" + std::to_string(j), &loops[j]->size, 8, 8);
+    for (const bh_base *b : index_checked) {
+        const int64_t n = b->nelem();
+        lit("const int64_t nb" + std::to_string(bid.at(b)), &n, 8, 8);
+    }
+    for (size_t j = 0; j < const_instrs.size(); ++j) {
+        const bh_constant &c = const_instrs[j]->constant;
+        const std::string cn = "c" + std::to_string(j);
+        if (c.type == bh_type::R123) {
+            lit("const uint64_t " + cn + "_start", &c.value.r123.start, 8, 8);
+            lit("const uint64_t " + cn + "_key", &c.value.r123.key, 8, 8);
+        } else {
+            const int size = bh_type_size(c.type);
+            const int align = bh_type_is_complex(c.type) ? size / 2 : size;
+            lit("const " + std::string(hip_type(c.type)) + " " + cn, &c.value, size, align);
+        }
+    }
+    if (kp.uses_index_check) {
+        params.push_back("unsigned int *__restrict__ bh_err");
+        ArgSpec a;
+        a.kind = ArgSpec::ERRFLAG_PTR;
+        kp.args.push_back(a);
+    }
+    s << "extern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.name << "(";
+    for (size_t i = 0; i < params.size(); ++i) s << (i ? ",\n    " : "") << params[i];
+    if (!extra.empty()) s << (params.empty() ? "" : ",\n    ") << extra;
+    s << ")";
+    return s.str();
+}
+
+// ---- MAP: grid-stride over the flattened leading parallel ranks
+void Writer::emit_map() {
+    const LoopB &l0 = unit._block_list.back().getLoop();
+    const uint64_t p = std::max<uint64_t>(1, jitk::parallel_ranks(l0).first);
+    const std::vector<const LoopB *> chain = jitk::get_first_loop_blocks(l0);
+    uint64_t total = 1;
+    for (uint64_t r = 0; r < p; ++r) total *= static_cast<uint64_t>(chain[r]->size);
+    kp.work_items = total;
+    const bool idx32 = total < (1ull << 31);
+    Scope ksc;
+    declare_scope(unit, ksc, 1);
+    body << "  int64_t tot_ = 1";
+    for (uint64_t r = 0; r < p; ++r) body << " * n" << lid.at(chain[r]);
+    body << ";\n";
+    body << "  for (int64_t f_ = (int64_t)blockIdx.x * BH_WG + threadIdx.x; f_ < tot_; f_ += (int64_t)gridDim.x * BH_WG) {\n";
+    if (p == 1) {
+        body << "    const int64_t i0 = f_;\n";
+    } else if (idx32) {
+        body << "    uint32_t r_ = (uint32_t)f_;\n";
+        for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r) {
+            body << "    const int64_t i" << r << " = (int64_t)(r_ % (uint32_t)n" << lid.at(chain[r]) << "); r_ /= (uint32_t)n"
+                 << lid.at(chain[r]) << ";\n";
+        }
+        body << "    const int64_t i0 = (int64_t)r_;\n";
+    } else {
+        body << "    int64_t r_ = f_;\n";
+        for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r)
+            body << "    const int64_t i" << r << " = r_ % n" << lid.at(chain[r]) << "; r_ /= n" << lid.at(chain[r]) << ";\n";
+        body << "    const int64_t i0 = r_;\n";
+    }
+    // Open the scopes of the parallel loops; the innermost one carries the body.
+    std::vector<Scope> scopes(p);
+    const Scope *parent = &ksc;
+    for (uint64_t r = 0; r < p; ++r) {
+        scopes[r].parent = parent;
+        body << std::string(4 + 2 * r, ' ') << "{\n";
+        declare_scope(*chain[r], scopes[r], 3 + r);
+        if (r == p - 1) emit_blocks(*chain[r], scopes[r], 3 + r);
+        parent = &scopes[r];
+    }
+    for (int64_t r = static_cast<int64_t>(p) - 1; r >= 0; --r) {
+        write_back(*chain[r], scopes[r], 3 + r);
+        body << std::string(4 + 2 * r, ' ') << "}\n";
+    }
+    body << "  }\n";
+    write_back(unit, ksc, 1);
+    kp.grid = std::max<uint64_t>(1, std::min<uint64_t>((total + kp.block - 1) / kp.block, cfg.max_grid));
+}
+
+// ---- FULL_REDUCE: rank-0 reductions with a deterministic single-launch grid combine
+void Writer::emit_full_reduce() {
+    const LoopB &l0 = unit._block_list.back().getLoop();
+    kp.work_items = static_cast<uint64_t>(l0.size);
+    std::vector<InstrPtr> sweeps;
+    for (const InstrPtr &i : jitk::allLocalInstr(l0))
+        if (bh_opcode_is_reduction(i->opcode) && i->sweep_axis() == 0) sweeps.push_back(i);
+    std::sort(sweeps.begin(), sweeps.end(), [](const InstrPtr &a, const InstrPtr &b) { return a->origin_id < b->origin_id; });
+
+    const uint64_t grid =
+        std::max<uint64_t>(1, std::min<uint64_t>((static_cast<uint64_t>(l0.size) + kp.block - 1) / kp.block, cfg.max_grid));
+    kp.grid = grid;
+
+    // Kernel scope: every rank-0 reduction output is a per-thread accumulator that starts at the identity.
+    Scope ksc;
+    struct Acc {
+        InstrPtr instr;
+        std::string var;
+        bh_type t;
+        int64_t part_off;
+        bool is_tmp;
+    };
+    std::vector<Acc> accs;
+    int64_t off = 0;
+    const std::set<bh_base *> ktmps = unit.getLocalTemps();
+    for (const InstrPtr &s : sweeps) {
+        const bh_view &v = s->operand[0];
+        const bh_type t = v.base->dtype();
+        Acc a;
+        a.instr = s;
+        a.t = t;
+        a.is_tmp = ktmps.count(v.base) > 0 || temps.count(v.base) > 0;
+        if (a.is_tmp) {
+            if (!ksc.tmps.count(v.base)) {
+                ksc.tmps.insert(v.base);
+                body << "  " << hip_type(t) << " t" << bid.at(v.base) << " = " << literal(jitk::sweep_identity(s->opcode, t)) << ";\n";
+            }
+            a.var = "t" + std::to_string(bid.at(v.base));
+        } else {
+            ksc.srep.insert(vkey(v));
+            a.var = "s" + std::to_string(bid.at(v.base)) + "_" + std::to_string(view_id(v));
+            body << "  " << hip_type(t) << " " << a.var << " = " << literal(jitk::sweep_identity(s->opcode, t)) << ";\n";
+        }
+        a.part_off = off;
+        off += ((static_cast<int64_t>(grid) * bh_type_size(t) + 255) / 256) * 256;
+        accs.push_back(a);
+    }
+    kp.scratch_bytes = off;
+    // Any other kernel-level temporaries.
+    declare_scope(unit, ksc, 1);
+    body << "  for (int64_t i0 = (int64_t)blockIdx.x * BH_WG + threadIdx.x; i0 < n" << lid.at(&l0)
+         << "; i0 += (int64_t)gridDim.x * BH_WG) {\n";
+    {
+        Scope sc;
+        sc.parent = &ksc;
+        declare_scope(l0, sc, 2);
+        emit_blocks(l0, sc, 2);
+        write_back(l0, sc, 2);
+    }
+    body << "  }\n";
+    body << "  __shared__ __attribute__((aligned(16))) unsigned char bh_lds_[BH_WAVES * 16];\n";
+    body << "  __shared__ bool bh_last_;\n";
+    for (size_t k = 0; k < accs.size(); ++k) {
+        const Acc &a = accs[k];
+        const std::string T = hip_type(a.t);
+        body << "  { auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
+             << combine_expr(a.instr->opcode, "x", "y") << "; };\n";
+        body << "    " << a.var << " = bh_block_reduce(" << a.var << ", op_, (" << T << " *)bh_lds_);\n";
+        body << "    if (threadIdx.x == 0) ((" << T << " *)(bh_scratch + " << a.part_off << "))[blockIdx.x] = " << a.var
+             << "; }\n";
+    }
+    body << "  __threadfence();\n  __syncthreads();\n";
+    body << "  if (threadIdx.x == 0) bh_last_ = (atomicAdd(bh_counter, 1u) == gridDim.x - 1);\n";
+    body << "  __syncthreads();\n";
+    body << "  if (!bh_last_) return;\n  __threadfence();\n";
+    for (size_t k = 0; k < accs.size(); ++k) {
+        const Acc &a = accs[k];
+        const std::string T = hip_type(a.t);
+        body << "  { auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
+             << combine_expr(a.instr->opcode, "x", "y") << "; };\n";
+        body << "    const " << T << " *p_ = (const " << T << " *)(bh_scratch + " << a.part_off << ");\n";
+        body << "    " << T << " v_ = " << literal(jitk::sweep_identity(a.instr->opcode, a.t)) << ";\n";
+        body << "    for (unsigned j_ = threadIdx.x; j_ < gridDim.x; j_ += BH_WG) v_ = op_(v_, p_[j_]);\n";
+        body << "    v_ = bh_block_reduce(v_, op_, (" << T << " *)bh_lds_);\n";
+        if (!a.is_tmp) {
+            const bh_view &v = a.instr->operand[0];
+            body << "    if (threadIdx.x == 0) a" << bid.at(v.base) << "[" << index(v) << "] = v_;\n";
+        }
+        body << "  }\n";
+    }
+    body << "  if (threadIdx.x == 0) *bh_counter = 0u;\n";
+}
+
+// ---- SCAN: decoupled look-back over tiles of BH_WG * items elements
+void Writer::emit_scan() {
+    const LoopB &l0 = unit._block_list.back().getLoop();
+    kp.work_items = static_cast<uint64_t>(l0.size);
+    InstrPtr acc;
+    for (const InstrPtr &i : jitk::allLocalInstr(l0))
+        if (bh_opcode_is_accumulate(i->opcode)) acc = i;
+    const bh_type t = acc->operand[0].base->dtype();
+    const std::string T = hip_type(t);
+    const int items = cfg.scan_items;
+    const int64_t tile_elems = static_cast<int64_t>(kp.block) * items;
+    const int64_t ntiles = (l0.size + tile_elems - 1) / tile_elems;
+    kp.grid = static_cast<uint64_t>(std::max<int64_t>(1, ntiles));
+    // Scratch layout: [epoch u32 | tile counter u32 | pad to 256] flags[ntiles] u32, agg[ntiles] T, inc[ntiles] T
+    const int64_t flags_off = 256;
+    const int64_t agg_off = flags_off + ((ntiles * 4 + 255) / 256) * 256;
+    const int64_t inc_off = agg_off + ((ntiles * bh_type_size(t) + 255) / 256) * 256;
+    kp.scratch_bytes = inc_off + ((ntiles * bh_type_size(t) + 255) / 256) * 256;
+    const std::string ident = literal(jitk::sweep_identity(acc->opcode, t));
+    scan_instr = acc.get();
+    scan_sink = "x_";
+
+    body << "  auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
+         << combine_expr(acc->opcode, "x", "y") << "; };\n";
+    body << "  unsigned int *bh_epoch_p = (unsigned int *)bh_scratch;\n";
+    body << "  unsigned int *bh_tiles_p = bh_epoch_p + 1;\n";
+    body << "  unsigned int *bh_flag = (unsigned int *)(bh_scratch + " << flags_off << ");\n";
+    body << "  " << T << " *bh_agg = (" << T << " *)(bh_scratch + " << agg_off << ");\n";
+    body << "  " << T << " *bh_inc = (" << T << " *)(bh_scratch + " << inc_off << ");\n";
+    body << "  __shared__ unsigned int bh_tile_s, bh_ep_s;\n";
+    body << "  __shared__ " << T << " bh_wtot_[BH_WAVES];\n";
+    body << "  __shared__ " << T << " bh_excl_s;\n";
+    body << "  if (threadIdx.x == 0) {\n";
+    body << "    bh_ep_s = __hip_atomic_load(bh_epoch_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "    bh_tile_s = atomicAdd(bh_tiles_p, 1u);\n  }\n";
+    body << "  __syncthreads();\n";
+    body << "  const int64_t tile_ = bh_tile_s;\n  const unsigned int ep_ = bh_ep_s * 4u;\n";
+    body << "  const int lane_ = threadIdx.x & 63, wave_ = threadIdx.x >> 6;\n";
+    body << "  const int64_t tb_ = tile_ * " << tile_elems << ";\n";
+    body << "  " << T << " v_[" << items << "];\n";
+    body << "  " << T << " carry_ = " << ident << ";\n";
+    Scope ksc;
+    declare_scope(unit, ksc, 1);
+    body << "  for (int j_ = 0; j_ < " << items << "; ++j_) {\n";
+    body << "    const int64_t i0 = tb_ + ((int64_t)wave_ * " << items << " + j_) * 64 + lane_;\n";
+    body << "    " << T << " x_ = " << ident << ";\n";
+    body << "    if (i0 < n" << lid.at(&l0) << ") {\n";
+    {
+        Scope sc;
+        sc.parent = &ksc;
+        declare_scope(l0, sc, 3);
+        emit_blocks(l0, sc, 3);
+        write_back(l0, sc, 3);
+    }
+    body << "    }\n";
+    body << "    x_ = op_(carry_, bh_wave_inclusive_scan(x_, op_));\n";
+    body << "    carry_ = bh_shfl<" << T << ">::idx(x_, 63);\n";
+    body << "    v_[j_] = x_;\n  }\n";
+    body << "  if (lane_ == 63) bh_wtot_[wave_] = carry_;\n  __syncthreads();\n";
+    body << "  " << T << " wpre_ = " << ident << ", ttot_ = " << ident << ";\n";
+    body << "  for (int w_ = 0; w_ < BH_WAVES; ++w_) { if (w_ < wave_) wpre_ = op_(wpre_, bh_wtot_[w_]); ttot_ = op_(ttot_, bh_wtot_[w_]); }\n";
+    body << "  if (threadIdx.x == 0) {\n";
+    body << "    " << T << " ex_ = " << ident << ";\n";
+    body << "    if (tile_ > 0) {\n";
+    body << "      bh_agg[tile_] = ttot_;\n";
+    body << "      __hip_atomic_store(&bh_flag[tile_], ep_ + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "      for (int64_t j_ = tile_ - 1;; --j_) {\n";
+    body << "        unsigned int f_;\n";
+    body << "        do { f_ = __hip_atomic_load(&bh_flag[j_], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }"
+            " while (f_ != ep_ + 1u && f_ != ep_ + 2u);\n";
+    body << "        if (f_ == ep_ + 2u) { ex_ = op_(bh_inc[j_], ex_); break; }\n";
+    body << "        ex_ = op_(bh_agg[j_], ex_);\n";
+    body << "      }\n    }\n";
+    body << "    bh_inc[tile_] = op_(ex_, ttot_);\n";
+    body << "    __hip_atomic_store(&bh_flag[tile_], ep_ + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "    bh_excl_s = ex_;\n";
+    body << "    if (tile_ == (int64_t)gridDim.x - 1) {\n";
+    body << "      *bh_tiles_p = 0u;\n";
+    body << "      __hip_atomic_store(bh_epoch_p, bh_ep_s + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "    }\n  }\n";
+    body << "  __syncthreads();\n";
+    body << "  const " << T << " pre_ = op_(bh_excl_s, wpre_);\n";
+    const bh_view &out = acc->operand[0];
+    body << "  for (int j_ = 0; j_ < " << items << "; ++j_) {\n";
+    body << "    const int64_t i0 = tb_ + ((int64_t)wave_ * " << items << " + j_) * 64 + lane_;\n";
+    body << "    if (i0 < n" << lid.at(&l0) << ") a" << bid.at(out.base) << "[" << index(out) << "] = op_(pre_, v_[j_]);\n";
+    body << "  }\n";
+    write_back(unit, ksc, 1);
+}
+
+// ---- SERIAL: one thread runs the whole loop nest
+void Writer::emit_serial() {
+    kp.block = 64;
+    kp.grid = 1;
+    body << "  if (blockIdx.x != 0 || threadIdx.x != 0) return;\n";
+    emit_body(unit, nullptr, 1);
+    kp.work_items = 1;
+}
+
+KernelPlan Writer::plan(Strategy s) {
+    kp.strategy = s;
+    kp.name = "bh_kernel";
+    if (s == Strategy::SERIAL) kp.block = 64;
+    switch (s) {
+        case Strategy::MAP: emit_map(); break;
+        case Strategy::FULL_REDUCE: emit_full_reduce(); break;
+        case Strategy::SCAN: emit_scan(); break;
+        case Strategy::SERIAL: emit_serial(); break;
+    }
+    std::string extra;
+    if (s == Strategy::FULL_REDUCE) {
+        extra = "unsigned char *__restrict__ bh_scratch,\n    unsigned int *__restrict__ bh_counter";
+    } else if (s == Strategy::SCAN) {
+        extra = "unsigned char *__restrict__ bh_scratch";
+    }
+    std::ostringstream src;
+    src << signature(extra) << " {\n" << body.str() << "}\n";
+    if (s == Strategy::FULL_REDUCE) {
+        ArgSpec a;
+        a.kind = ArgSpec::SCRATCH_PTR;
+        kp.args.push_back(a);
+        ArgSpec c;
+        c.kind = ArgSpec::COUNTER_PTR;
+        kp.args.push_back(c);
+    } else if (s == Strategy::SCAN) {
+        ArgSpec a;
+        a.kind = ArgSpec::SCRATCH_PTR;
+        kp.args.push_back(a);
+    }
+    kp.source = src.str();
+    return kp;
+}
+
+bool scan_compatible(const LoopB &l0, const InstrPtr &acc) {
+    if (!(acc->opcode == BH_ADD_ACCUMULATE || acc->opcode == BH_MULTIPLY_ACCUMULATE)) return false;
+    if (!l0.isInnermost()) return false;
+    const bh_base *out = acc->operand[0].base;
+    for (const InstrPtr &i : jitk::allInstr(l0)) {
+        if (i == acc) continue;
+        for (const bh_view &v : i->operand)
+            if (!v.isConstant() && v.base == out) return false;
+    }
+    for (const bh_view &v : acc->operand)
+        if (&v != &acc->operand[0] && !v.isConstant() && v.base == out) return false;
+    return true;
+}
+
+} // namespace
+
+Strategy classify_root_block(const LoopB &l0) {
+    if (l0._sweeps.empty()) return Strategy::MAP;
+    const auto local = jitk::allLocalInstr(l0);
+    auto is_local = [&](const InstrPtr &s) { return std::find(local.begin(), local.end(), s) != local.end(); };
+    bool all_red_local = true;
+    for (const InstrPtr &s : l0._sweeps)
+        if (!bh_opcode_is_reduction(s->opcode) || !is_local(s)) all_red_local = false;
+    if (all_red_local) return Strategy::FULL_REDUCE;
+    if (l0._sweeps.size() == 1) {
+        const InstrPtr &s = *l0._sweeps.begin();
+        if (bh_opcode_is_accumulate(s->opcode) && is_local(s) && scan_compatible(l0, s)) return Strategy::SCAN;
+    }
+    return Strategy::SERIAL;
+}
+
+bool root_merge_allowed(const LoopB &l1, const LoopB &l2) {
+    if (l1.rank != 0 || l2.rank != 0) return true;
+    size_t nsweeps = l1._sweeps.size() + l2._sweeps.size();
+    bool acc = false;
+    for (const InstrPtr &s : l1._sweeps) acc |= bh_opcode_is_accumulate(s->opcode) != 0;
+    for (const InstrPtr &s : l2._sweeps) acc |= bh_opcode_is_accumulate(s->opcode) != 0;
+    if (acc && nsweeps > 1) return false;
+    if (acc && (!l1.isInnermost() || !l2.isInnermost())) return false;
+    return true;
+}
+
+namespace {
+// A block that only writes sweep identities (BH_IDENTITY of a constant).
+bool identity_only(const LoopB &l) {
+    const auto all = jitk::allInstr(l);
+    if (all.empty()) return false;
+    for (const InstrPtr &i : all)
+        if (i->opcode != BH_IDENTITY || !i->operand[1].isConstant()) return false;
+    return true;
+}
+
+// True when every identity in `ids` initialises the output of a sweep of `l0`.
+bool identities_of(const std::vector<Block> &ids, const LoopB &l0) {
+    std::set<const bh_base *> outs;
+    for (const InstrPtr &s : l0._sweeps) outs.insert(s->operand[0].base);
+    for (const Block &b : ids)
+        for (const InstrPtr &i : jitk::allInstr(b))
+            if (!outs.count(i->operand[0].base)) return false;
+    return true;
+}
+} // namespace
+
+std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg) {
+    std::vector<KernelPlan> ret;
+    auto emit = [&](LoopB unit, Strategy s) {
+        unit.metadataUpdate();
+        Writer w(unit, cfg);
+        ret.push_back(w.plan(s));
+    };
+    // Each top-level loop is one launch.  Identity writers that precede a sweeping
+    // loop are folded into it when its skeleton seeds the sweep with the identity
+    // itself (FULL_REDUCE, SCAN); otherwise they run first.
+    std::vector<Block> pending;
+    for (const Block &b : kernel._block_list) {
+        if (b.isInstr()) {
+            pending.push_back(b);
+            continue;
+        }
+        if (b.isSystemOnly()) continue;
+        if (identity_only(b.getLoop())) {
+            pending.push_back(b);
+            continue;
+        }
+        const LoopB &l0 = b.getLoop();
+        Strategy s = classify_root_block(l0);
+        const bool seeds_itself = s == Strategy::FULL_REDUCE || s == Strategy::SCAN;
+        std::vector<Block> scalar_ids;
+        if (!(seeds_itself && identities_of(pending, l0))) {
+            for (const Block &p : pending) {
+                if (p.isInstr()) {
+                    scalar_ids.push_back(p);
+                } else {
+                    emit(LoopB(-1, 1, {p}), Strategy::MAP);
+                }
+            }
+        }
+        pending.clear();
+        if (s == Strategy::MAP && l0.size == 0) continue;
+        LoopB unit(-1, 1, scalar_ids);
+        unit._block_list.push_back(b);
+        if (s == Strategy::MAP && unit._block_list.size() > 1) s = Strategy::SERIAL;
+        emit(unit, s);
+    }
+    for (const Block &p : pending) {
+        if (p.isInstr()) {
+            emit(LoopB(-1, 1, {p}), Strategy::SERIAL);
+        } else {
+            emit(LoopB(-1, 1, {p}), Strategy::MAP);
+        }
+    }
+    return ret;
+}
+
+} // namespace vehip
+} // namespace bohrium

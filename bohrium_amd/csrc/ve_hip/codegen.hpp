@@ -1,0 +1,77 @@
+/* HIP kernel writer for the hip vector engine.
+ *
+ * Each top-level block of a jitk kernel becomes one launch of one of these
+ * skeletons, into which the block's scalar body is spliced:
+ *   MAP          grid-stride loop over the flattened leading parallel ranks;
+ *                inner ranks (and inner sweeps) run as sequential loops.
+ *   FULL_REDUCE  rank-0 reductions: per-thread accumulators, wave64 butterfly,
+ *                LDS across waves, per-workgroup partials and a last-workgroup
+ *                combine in workgroup order (deterministic, one launch).
+ *   SCAN         rank-0 add/multiply accumulate: wave64 shuffle scan, LDS
+ *                across waves and a decoupled look-back across workgroups.
+ *   SERIAL       anything else, executed by one thread (always correct).
+ * Per-opcode scalar semantics restate the reference's write_operation
+ * (core/jitk/instruction.cpp:58-544) and engine.cpp:251-336 special forms.
+ */
+#pragma once
+
+#include <cstdint>
+#include <set>
+#include <string>
+#include <vector>
+#include <bohrium/jitk/block.hpp>
+
+namespace bohrium {
+namespace vehip {
+
+enum class Strategy { MAP, FULL_REDUCE, SCAN, SERIAL };
+const char *strategy_name(Strategy s);
+
+/* One kernel argument: either literal bytes known at plan time, or a pointer
+ * the engine resolves at launch time. */
+struct ArgSpec {
+    enum Kind { LITERAL, BASE_PTR, SCRATCH_PTR, COUNTER_PTR, ERRFLAG_PTR } kind = LITERAL;
+    const bh_base *base = nullptr;
+    int64_t scratch_offset = 0;  // byte offset into the engine's scratch area
+    std::vector<uint8_t> bytes;  // LITERAL value
+    int size = 8;
+    int align = 8;
+};
+
+struct KernelPlan {
+    Strategy strategy = Strategy::SERIAL;
+    std::string source;  // kernel source without the prelude
+    std::string name;
+    std::vector<ArgSpec> args;
+    uint64_t grid = 1;
+    uint32_t block = 256;
+    /* Bases that must be resident on the device for this launch, and those it writes. */
+    std::vector<bh_base *> params;
+    std::set<bh_base *> written;
+    /* Scratch bytes (partials / look-back state) this launch needs. */
+    int64_t scratch_bytes = 0;
+    bool uses_index_check = false;
+    uint64_t work_items = 0;  // elements of the iteration space (for statistics / cost model features)
+};
+
+/* Hooks the engine supplies to the writer. */
+struct WriterConfig {
+    int max_grid = 2048;     // workgroups for grid-stride skeletons
+    int scan_items = 8;      // elements per thread per scan tile
+    uint32_t epoch = 1;      // look-back flag generation (changes every launch)
+};
+
+/* Split a jitk kernel (rank -1 LoopB) into launch plans. */
+std::vector<KernelPlan> plan_kernel(const jitk::LoopB &kernel, const WriterConfig &cfg);
+
+/* Which skeleton runs a rank-0 block (used by the fuser to keep blocks on parallel skeletons). */
+Strategy classify_root_block(const jitk::LoopB &l0);
+
+/* True when merging two root blocks keeps the result on a parallel skeleton. */
+bool root_merge_allowed(const jitk::LoopB &l1, const jitk::LoopB &l2);
+
+/* HIP spelling of an element type inside generated kernels. */
+const char *hip_type(bh_type t);
+
+} // namespace vehip
+} // namespace bohrium

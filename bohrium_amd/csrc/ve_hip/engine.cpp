@@ -1,0 +1,590 @@
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <iomanip>
+#include <sstream>
+#include <stdexcept>
+
+#include <bohrium/bh_main_memory.hpp>
+#include "../../../include/bhhip.h"
+
+namespace bohrium {
+namespace vehip {
+
+namespace {
+const char *kPrelude =
+#include "kernel_prelude.inc"
+    ;
+
+void check(int rc, const char *what) {
+    if (rc != 0) throw std::runtime_error(std::string("hip engine: ") + what + ": " + bhhip_last_error());
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+uint64_t fnv1a(const std::string &s) {
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string default_cache_dir() {
+    const char *xdg = std::getenv("XDG_CACHE_HOME");
+    const char *home = std::getenv("HOME");
+    if (xdg != nullptr && *xdg) return std::string(xdg) + "/bohrium_amd";
+    if (home != nullptr && *home) return std::string(home) + "/.cache/bohrium_amd";
+    return "/tmp/bohrium_amd_cache";
+}
+} // namespace
+
+// ------------------------------------------------------------------ pool
+
+void *DevicePool::alloc(uint64_t bytes) {
+    bytes = std::max<uint64_t>(256, (bytes + 255) / 256 * 256);
+    auto it = _free.find(bytes);
+    if (it != _free.end()) {
+        void *p = it->second;
+        _free.erase(it);
+        _cached -= bytes;
+        ++hits;
+        return p;
+    }
+    ++misses;
+    void *p = nullptr;
+    if (bhhip_malloc(&p, bytes) != 0) {
+        trim(0);
+        check(bhhip_malloc(&p, bytes), "device allocation");
+    }
+    allocated += bytes;
+    peak = std::max(peak, allocated);
+    return p;
+}
+
+void DevicePool::release(void *ptr, uint64_t bytes) {
+    if (ptr == nullptr) return;
+    bytes = std::max<uint64_t>(256, (bytes + 255) / 256 * 256);
+    _free.emplace(bytes, ptr);
+    _cached += bytes;
+    if (_cached > limit) trim(limit / 2);
+}
+
+void DevicePool::trim(uint64_t keep) {
+    while (_cached > keep && !_free.empty()) {
+        auto it = std::prev(_free.end());  // largest first
+        bhhip_synchronize();
+        bhhip_free(it->second);
+        _cached -= it->first;
+        allocated -= it->first;
+        _free.erase(it);
+    }
+}
+
+DevicePool::~DevicePool() {
+    for (auto &e : _free) bhhip_free(e.second);
+}
+
+// ------------------------------------------------------------------ engine
+
+EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
+    const ConfigParser &cfg = comp.config;
+    const char *dry_env = std::getenv("BH_HIP_DRY_RUN");
+    dry_run = cfg.defaultGet<bool>("dry_run", false) || (dry_env != nullptr && std::string(dry_env) == "1");
+    if (dry_run) {
+        check(bhhip_set_arch(cfg.defaultGet<std::string>("dry_run_arch", "gfx950").c_str()), "set arch");
+    } else {
+        check(bhhip_init(cfg.defaultGet<int>("device_number", 0)), "init");
+    }
+    fusion_config.pre_fuser = cfg.defaultGet<std::string>("pre_fuser", "lossy");
+    fusion_config.fuser_list =
+        cfg.defaultGetList("fuser_list", {"greedy", "push_reductions_inwards", "collapse_redundant_axes"});
+    fusion_config.greedy_threshold = cfg.defaultGet<uint64_t>("greedy_threshold", 10000);
+    fusion_config.avoid_rank0_sweep = false;
+    fusion_config.merge_filter = [](const jitk::LoopB &a, const jitk::LoopB &b) { return root_merge_allowed(a, b); };
+    wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
+    wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
+    resident = cfg.defaultGet<bool>("resident", true);
+    profile = cfg.defaultGet<bool>("prof", false);
+    cache_dir = cfg.defaultGet<std::string>("cache_dir", "");
+    if (cache_dir.empty() || cache_dir == "NONE") cache_dir = default_cache_dir();
+    if (cache_dir == "off") cache_dir.clear();
+    pool.limit = cfg.defaultGet<uint64_t>("malloc_cache_limit_bytes", 64ull << 30);
+    if (dry_run) return;
+    void *c = nullptr;
+    check(bhhip_malloc(&c, 256), "allocation of counters");
+    check(bhhip_memzero(c, 256), "zeroing counters");
+    counters = static_cast<unsigned int *>(c);
+}
+
+EngineHip::~EngineHip() {
+    if (dry_run) return;
+    bhhip_synchronize();
+    for (auto &kv : bufs)
+        if (!kv.second.external) pool.release(kv.second.ptr, kv.second.bytes);
+    bufs.clear();
+    if (reduce_scratch) bhhip_free(reduce_scratch);
+    if (scan_scratch) bhhip_free(scan_scratch);
+    if (counters) bhhip_free(counters);
+}
+
+DeviceBuffer &EngineHip::buffer(bh_base *base) {
+    auto it = bufs.find(base);
+    if (it != bufs.end()) return it->second;
+    DeviceBuffer &b = bufs[base];
+    b.host_valid = base->getDataPtr() != nullptr;
+    return b;
+}
+
+void EngineHip::ensureResident(bh_base *base) {
+    DeviceBuffer &b = buffer(base);
+    if (dry_run) return;
+    if (b.ptr == nullptr) {
+        b.bytes = static_cast<uint64_t>(base->nbytes());
+        b.ptr = pool.alloc(b.bytes);
+    }
+    if (!b.dev_valid && b.host_valid && base->getDataPtr() != nullptr) {
+        const auto t = std::chrono::steady_clock::now();
+        check(bhhip_memcpy_h2d(b.ptr, base->getDataPtr(), b.bytes), "copy to device");
+        stats.h2d_bytes += b.bytes;
+        stats.time_copy += seconds_since(t);
+        b.dev_valid = true;
+    }
+}
+
+void EngineHip::copyToHost(bh_base *base) {
+    auto it = bufs.find(base);
+    if (it == bufs.end()) return;
+    DeviceBuffer &b = it->second;
+    if (dry_run) {
+        bh_data_malloc(base);
+        return;
+    }
+    if (!b.dev_valid || b.host_valid || b.ptr == nullptr) return;
+    const auto t = std::chrono::steady_clock::now();
+    bh_data_malloc(base);
+    check(bhhip_memcpy_d2h(base->getDataPtr(), b.ptr, b.bytes), "copy to host");
+    stats.d2h_bytes += b.bytes;
+    stats.time_copy += seconds_since(t);
+    b.host_valid = true;
+}
+
+void EngineHip::copyAllToHost() {
+    for (auto &kv : bufs) copyToHost(const_cast<bh_base *>(kv.first));
+}
+
+void EngineHip::release(bh_base *base) {
+    auto it = bufs.find(base);
+    if (it != bufs.end()) {
+        if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
+        bufs.erase(it);
+    }
+    bh_data_free(base);
+}
+
+void EngineHip::ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need, bool zero) {
+    if (need <= have) return;
+    const uint64_t bytes = std::max<uint64_t>(need, 1 << 20);
+    if (ptr) {
+        check(bhhip_synchronize(), "synchronize");
+        bhhip_free(ptr);
+    }
+    void *p = nullptr;
+    check(bhhip_malloc(&p, bytes), "scratch allocation");
+    if (zero) check(bhhip_memzero(p, bytes), "scratch zeroing");
+    ptr = static_cast<unsigned char *>(p);
+    have = bytes;
+}
+
+void EngineHip::launch(const KernelPlan &plan) {
+    auto t = std::chrono::steady_clock::now();
+    void *fn = nullptr;
+    if (dry_run) {
+        check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), nullptr),
+              "kernel compilation");
+        stats.time_compile += seconds_since(t);
+        char hex[32];
+        std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(fnv1a(plan.source)));
+        dry_run_log.push_back(std::string(strategy_name(plan.strategy)) + " " + std::to_string(plan.grid) + " " +
+                              std::to_string(plan.block) + " " + hex);
+        ++stats.launches;
+        ++stats.launches_by_strategy[strategy_name(plan.strategy)];
+        return;
+    }
+    check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &fn),
+          "kernel compilation");
+    stats.time_compile += seconds_since(t);
+
+    t = std::chrono::steady_clock::now();
+    for (bh_base *b : plan.params) ensureResident(b);
+    if (plan.strategy == Strategy::FULL_REDUCE) ensureScratch(reduce_scratch, reduce_scratch_bytes, plan.scratch_bytes, false);
+    if (plan.strategy == Strategy::SCAN) ensureScratch(scan_scratch, scan_scratch_bytes, plan.scratch_bytes, true);
+
+    std::vector<uint8_t> buf;
+    buf.reserve(512);
+    auto put = [&](const void *p, int size, int align) {
+        while (buf.size() % static_cast<size_t>(align)) buf.push_back(0);
+        const uint8_t *b = static_cast<const uint8_t *>(p);
+        buf.insert(buf.end(), b, b + size);
+    };
+    for (const ArgSpec &a : plan.args) {
+        void *ptr = nullptr;
+        switch (a.kind) {
+            case ArgSpec::LITERAL: put(a.bytes.data(), a.size, a.align); continue;
+            case ArgSpec::BASE_PTR: ptr = bufs.at(a.base).ptr; break;
+            case ArgSpec::SCRATCH_PTR:
+                ptr = plan.strategy == Strategy::SCAN ? scan_scratch : reduce_scratch;
+                break;
+            case ArgSpec::COUNTER_PTR: ptr = counters; break;
+            case ArgSpec::ERRFLAG_PTR: ptr = counters + 1; break;
+        }
+        put(&ptr, 8, 8);
+    }
+    while (buf.size() % 8) buf.push_back(0);
+
+    void *ev0 = nullptr, *ev1 = nullptr;
+    if (profile) {
+        bhhip_event_create(&ev0);
+        bhhip_event_create(&ev1);
+        bhhip_event_record(ev0);
+    }
+    check(bhhip_launch(fn, plan.grid, plan.block, buf.data(), buf.size()), "kernel launch");
+    ++stats.launches;
+    ++stats.launches_by_strategy[strategy_name(plan.strategy)];
+    for (bh_base *b : plan.written) {
+        DeviceBuffer &d = buffer(b);
+        d.dev_valid = true;
+        d.host_valid = false;
+    }
+    if (profile) {
+        bhhip_event_record(ev1);
+        float ms = 0;
+        bhhip_event_elapsed_ms(ev0, ev1, &ms);
+        bhhip_event_destroy(ev0);
+        bhhip_event_destroy(ev1);
+        stats.device_ms_by_strategy[strategy_name(plan.strategy)] += ms;
+        char hex[32];
+        std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(fnv1a(plan.source)));
+        last_launches.push_back({strategy_name(plan.strategy), plan.work_items, plan.grid, ms, hex});
+    }
+    stats.time_launch += seconds_since(t);
+}
+
+void EngineHip::checkErrors() {
+    if (dry_run) return;
+    unsigned int flag = 0;
+    check(bhhip_memcpy_d2h(&flag, counters + 1, sizeof(flag)), "reading the error flag");
+    if (flag != 0) {
+        const unsigned int zero = 0;
+        check(bhhip_memcpy_h2d(counters + 1, &zero, sizeof(zero)), "clearing the error flag");
+        throw std::out_of_range("hip engine: gather/scatter index out of bounds");
+    }
+}
+
+void EngineHip::handleExecution(BhIR *bhir) {
+    const auto t_total = std::chrono::steady_clock::now();
+    ++stats.flushes;
+    last_launches.clear();
+    std::set<bh_base *> frees;
+    std::vector<bh_instruction *> instr_list = jitk::remove_non_computed_system_instr(bhir->instr_list, frees);
+    for (bh_base *b : frees) release(b);
+    jitk::set_constructor_flag(instr_list);
+
+    auto t = std::chrono::steady_clock::now();
+    std::vector<jitk::LoopB> kernels = jitk::get_kernel_list(instr_list, fusion_config);
+    stats.time_fuse += seconds_since(t);
+
+    bool index_checks = false;
+    static const bool debug = std::getenv("BH_HIP_DEBUG") != nullptr;
+    for (const jitk::LoopB &kernel : kernels) {
+        ++stats.kernels;
+        if (debug) std::fprintf(stderr, "[hip] kernel:\n%s", kernel.pprint().c_str());
+        if (!kernel.isSystemOnly()) {
+            t = std::chrono::steady_clock::now();
+            std::vector<KernelPlan> plans = plan_kernel(kernel, wcfg);
+            stats.time_codegen += seconds_since(t);
+            for (const KernelPlan &p : plans) {
+                if (debug) std::fprintf(stderr, "[hip] launch %s grid %llu\n%s\n", strategy_name(p.strategy),
+                                        static_cast<unsigned long long>(p.grid), std::getenv("BH_HIP_DEBUG")[0] == '2' ? p.source.c_str() : "");
+                launch(p);
+                index_checks |= p.uses_index_check;
+            }
+        }
+        for (bh_base *b : kernel.getAllFrees()) release(b);
+    }
+    if (index_checks) checkErrors();
+    for (bh_base *b : bhir->getSyncs()) {
+        copyToHost(b);
+        if (!resident) {
+            auto it = bufs.find(b);
+            if (it != bufs.end()) {
+                if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
+                bufs.erase(it);
+            }
+        }
+    }
+    stats.time_total += seconds_since(t_total);
+}
+
+void EngineHip::handleExtmethod(BhIR *bhir) {
+    std::vector<bh_instruction> instr_list;
+    for (bh_instruction &instr : bhir->instr_list) {
+        auto ext = comp.extmethods.find(instr.opcode);
+        if (ext != comp.extmethods.end()) {
+            BhIR b(std::move(instr_list), bhir->getSyncs());
+            handleExecution(&b);
+            instr_list.clear();
+            for (const bh_view &v : instr.operand)
+                if (!v.isConstant()) ensureResident(v.base);
+            ext->second.execute(&instr, this);
+            if (!instr.operand.empty() && !instr.operand[0].isConstant()) {
+                DeviceBuffer &d = buffer(instr.operand[0].base);
+                d.dev_valid = true;
+                d.host_valid = false;
+            }
+        } else if (comp.child_extmethods.count(instr.opcode)) {
+            throw std::runtime_error("hip engine: extension method is only known to a child component");
+        } else {
+            instr_list.push_back(instr);
+        }
+    }
+    bhir->instr_list = instr_list;
+}
+
+void *EngineHip::getMemoryPointer(bh_base &base, bool copy2host, bool force_alloc, bool nullify) {
+    bh_base *b = &base;
+    if (!copy2host) {
+        ensureResident(b);
+        if (dry_run) return nullptr;
+        return bufs.at(b).ptr;
+    }
+    copyToHost(b);
+    if (force_alloc && b->getDataPtr() == nullptr) {
+        bh_data_malloc(b);
+        buffer(b);
+    }
+    void *ret = b->getDataPtr();
+    auto it = bufs.find(b);
+    if (!resident && it != bufs.end()) {
+        if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
+        bufs.erase(it);
+        it = bufs.end();
+    }
+    if (nullify) {
+        b->resetDataPtr(nullptr);
+        if (it != bufs.end()) it->second.host_valid = false;
+    }
+    return ret;
+}
+
+void EngineHip::setMemoryPointer(bh_base *base, bool host_ptr, void *mem) {
+    DeviceBuffer &d = buffer(base);
+    if (host_ptr) {
+        if (base->getDataPtr() != nullptr && base->getDataPtr() != mem) bh_data_free(base);
+        base->resetDataPtr(mem);
+        d.host_valid = mem != nullptr;
+        d.dev_valid = false;
+    } else {
+        if (d.ptr && !d.external) pool.release(d.ptr, d.bytes);
+        d.ptr = mem;
+        d.bytes = static_cast<uint64_t>(base->nbytes());
+        d.external = true;
+        d.dev_valid = true;
+        d.host_valid = false;
+    }
+}
+
+std::string EngineHip::info() const {
+    std::ostringstream ss;
+    ss << "----\nHIP vector engine:\n  Device: " << bhhip_device_info() << "\n  Kernel cache: "
+       << (cache_dir.empty() ? "memory only" : cache_dir) << "\n  Fuser list:";
+    for (const std::string &f : fusion_config.fuser_list) ss << " " << f;
+    ss << "\n  Resident arrays: " << (resident ? "yes" : "no") << "\n";
+    return ss.str();
+}
+
+std::string EngineHip::statistics() const {
+    uint64_t compiles = 0, mhits = 0, dhits = 0;
+    bhhip_compile_stats(&compiles, &mhits, &dhits);
+    std::ostringstream ss;
+    ss << std::fixed << std::setprecision(6);
+    ss << "flushes: " << stats.flushes << "\nkernels: " << stats.kernels << "\nlaunches: " << stats.launches << "\n";
+    for (const auto &kv : stats.launches_by_strategy) ss << "launches_" << kv.first << ": " << kv.second << "\n";
+    ss << "compiles: " << compiles << "\ncompile_memory_hits: " << mhits << "\ncompile_disk_hits: " << dhits << "\n";
+    ss << "h2d_bytes: " << stats.h2d_bytes << "\nd2h_bytes: " << stats.d2h_bytes << "\n";
+    ss << "time_fuse: " << stats.time_fuse << "\ntime_codegen: " << stats.time_codegen
+       << "\ntime_compile: " << stats.time_compile << "\ntime_launch: " << stats.time_launch
+       << "\ntime_copy: " << stats.time_copy << "\ntime_total: " << stats.time_total << "\n";
+    for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
+    ss << "pool_allocated: " << pool.allocated << "\npool_peak: " << pool.peak << "\npool_hits: " << pool.hits
+       << "\npool_misses: " << pool.misses << "\n";
+    return ss.str();
+}
+
+// ------------------------------------------------------------------ component
+
+namespace {
+
+// Advance views that slide between repeats (reference include/bohrium/jitk/engines/dyn_view.hpp:21-67).
+void slide_views(BhIR *bhir) {
+    for (bh_instruction &instr : bhir->instr_list) {
+        for (bh_view &view : instr.operand) {
+            if (!view.hasSlide()) continue;
+            for (const bh_slide_dim &dim : view.slides.dims) {
+                if (dim.step_delay == 1 || (view.slides.iteration_counter % dim.step_delay == dim.step_delay - 1)) {
+                    if (dim.stride) {
+                        int64_t change = dim.offset_change * dim.stride;
+                        const int64_t max_rel = dim.stride * dim.shape;
+                        int64_t rel = view.start % (dim.stride * dim.shape);
+                        rel += change;
+                        if (rel < 0) {
+                            change += max_rel;
+                        } else if (rel >= max_rel) {
+                            change -= max_rel;
+                        }
+                        view.start += change;
+                        auto it = view.slides.resets.find(dim.rank);
+                        if (it != view.slides.resets.end()) {
+                            const int64_t reset_at = it->second.first;
+                            int64_t &since = it->second.second;
+                            since += change;
+                            if (view.slides.iteration_counter > 0 &&
+                                (view.slides.iteration_counter / dim.step_delay) % reset_at == reset_at - 1) {
+                                view.start -= since;
+                                since = 0;
+                                view.shape[dim.rank] -= reset_at * dim.shape_change;
+                            }
+                        }
+                    }
+                    view.shape[dim.rank] += dim.shape_change;
+                    if (view.shape[dim.rank] < 0) view.shape[dim.rank] = 0;
+                }
+            }
+            view.slides.iteration_counter += 1;
+        }
+    }
+}
+
+class Impl : public component::ComponentVE {
+public:
+    EngineHip engine;
+
+    explicit Impl(int stack_level) : ComponentVE(stack_level), engine(*this) {}
+    ~Impl() override = default;
+
+    void execute(BhIR *bhir) override {
+        if (disabled) {
+            child.execute(bhir);
+            return;
+        }
+        bh_base *cond = bhir->getRepeatCondition();
+        for (uint64_t i = 0; i < bhir->getNRepeats(); ++i) {
+            engine.handleExtmethod(bhir);
+            engine.handleExecution(bhir);
+            if (cond != nullptr) {
+                engine.copyToHost(cond);
+                if (cond->getDataPtr() != nullptr && !static_cast<bool *>(cond->getDataPtr())[0]) break;
+            }
+            slide_views(bhir);
+        }
+    }
+
+    void extmethod(const std::string &name, bh_opcode opcode) override {
+        try {
+            extmethods.insert(std::make_pair(opcode, extmethod::ExtmethodFace(config.defaultGetList("libs", {}), name)));
+        } catch (const std::out_of_range &) {
+            if (!child.initiated()) throw;
+            child.extmethod(name, opcode);
+            child_extmethods.insert(opcode);
+        }
+    }
+
+    std::string message(const std::string &msg) override {
+        std::ostringstream ss;
+        if (msg == "statistic_enable_and_reset") {
+            engine.resetStatistics();
+            engine.profile = true;
+        } else if (msg == "statistic") {
+            ss << engine.statistics();
+        } else if (msg == "info") {
+            ss << engine.info();
+        } else if (msg == "GPU: disable") {
+            if (!child.initiated()) throw std::runtime_error("hip engine: cannot disable the GPU without a child engine");
+            engine.copyAllToHost();
+            disabled = true;
+        } else if (msg == "GPU: enable") {
+            disabled = false;
+        } else if (msg == "profile: on") {
+            engine.profile = true;
+        } else if (msg == "profile: off") {
+            engine.profile = false;
+        } else if (msg == "plan log") {
+            for (const std::string &l : engine.dry_run_log) ss << l << "\n";
+            engine.dry_run_log.clear();
+        } else if (msg == "launches") {
+            for (const auto &r : engine.last_launches)
+                ss << r.strategy << " " << r.work_items << " " << r.grid << " " << r.device_ms << " " << r.kernel_hash
+                   << "\n";
+        } else if (msg.rfind("fuser_list:", 0) == 0) {
+            std::vector<std::string> list;
+            std::istringstream in(msg.substr(11));
+            std::string item;
+            while (std::getline(in, item, ',')) {
+                item.erase(0, item.find_first_not_of(' '));
+                item.erase(item.find_last_not_of(' ') + 1);
+                if (!item.empty()) list.push_back(item);
+            }
+            engine.fusion_config.fuser_list = list;
+        }
+        if (child.initiated()) ss << child.message(msg);
+        return ss.str();
+    }
+
+    void *getMemoryPointer(bh_base &base, bool copy2host, bool force_alloc, bool nullify) override {
+        return engine.getMemoryPointer(base, copy2host, force_alloc, nullify);
+    }
+
+    void setMemoryPointer(bh_base *base, bool host_ptr, void *mem) override {
+        engine.setMemoryPointer(base, host_ptr, mem);
+    }
+
+    void memCopy(bh_view &src, bh_view &dst, const std::string &) override {
+        // A contiguous whole-base copy between two bases.
+        if (src.base->nbytes() != dst.base->nbytes()) throw std::runtime_error("memCopy: size mismatch");
+        void *s = engine.getMemoryPointer(*src.base, false, false, false);
+        void *d = engine.getMemoryPointer(*dst.base, false, false, false);
+        check(bhhip_memcpy_d2d(d, s, static_cast<size_t>(src.base->nbytes())), "memCopy");
+        engine.setMemoryPointer(dst.base, false, d);
+    }
+
+    void *getDeviceContext() override { return nullptr; }
+    void setDeviceContext(void *) override {}
+
+    std::string userKernel(const std::string &kernel, std::vector<bh_view> &operand_list, const std::string &compile_cmd,
+                           const std::string &tag, const std::string &param) override {
+        if (tag != "hip") {
+            if (!child.initiated()) throw std::runtime_error("No backend with tag \"" + tag + "\" found");
+            for (const bh_view &v : operand_list) engine.copyToHost(v.base);
+            return child.userKernel(kernel, operand_list, compile_cmd, tag, param);
+        }
+        throw std::runtime_error("hip engine: userKernel is not implemented");
+    }
+};
+
+} // namespace
+
+} // namespace vehip
+} // namespace bohrium
+
+extern "C" bohrium::component::ComponentImpl *create(int stack_level) {
+    return new bohrium::vehip::Impl(stack_level);
+}
+
+extern "C" void destroy(bohrium::component::ComponentImpl *self) { delete self; }

@@ -1,0 +1,110 @@
+/* The hip vector engine: BhIR -> fused kernels -> hiprtc -> launches on one
+ * HIP stream, with arrays kept resident in device memory between flushes.
+ *
+ * Per-flush flow mirrors the reference GPU engine's handleExecution
+ * (include/bohrium/jitk/engines/engine_gpu.hpp:81-160): drop non-computing
+ * system instructions and free what they release, set constructor flags, fuse
+ * into kernels, run each kernel, release arrays freed inside it, copy sync'ed
+ * arrays to the host.  Differences by design: every kernel runs on the device
+ * (no CPU offload), launches are stream-ordered without a per-launch
+ * synchronize, and a sync leaves the device copy valid.
+ */
+#pragma once
+
+#include <map>
+#include <set>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#include <bohrium/bh_component.hpp>
+#include <bohrium/jitk/fuser.hpp>
+#include "codegen.hpp"
+
+namespace bohrium {
+namespace vehip {
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+    bool dev_valid = false;   // device holds the newest data
+    bool host_valid = false;  // host buffer holds the newest data
+    bool external = false;    // device memory supplied by the caller (not pooled)
+};
+
+/* Size-keyed reuse of device allocations (hipMalloc is far slower than a lookup). */
+class DevicePool {
+    std::multimap<uint64_t, void *> _free;
+    uint64_t _cached = 0;
+
+public:
+    uint64_t limit = 64ull << 30;
+    uint64_t allocated = 0, peak = 0, hits = 0, misses = 0;
+    void *alloc(uint64_t bytes);
+    void release(void *ptr, uint64_t bytes);
+    void trim(uint64_t keep);
+    ~DevicePool();
+};
+
+struct EngineStats {
+    uint64_t flushes = 0, kernels = 0, launches = 0, h2d_bytes = 0, d2h_bytes = 0;
+    std::map<std::string, uint64_t> launches_by_strategy;
+    double time_fuse = 0, time_codegen = 0, time_compile = 0, time_launch = 0, time_copy = 0, time_total = 0;
+    /* Per-launch device time (ms) keyed by strategy, recorded when profiling is enabled. */
+    std::map<std::string, double> device_ms_by_strategy;
+};
+
+class EngineHip {
+public:
+    explicit EngineHip(component::ComponentVE &comp);
+    ~EngineHip();
+
+    void handleExecution(BhIR *bhir);
+    void handleExtmethod(BhIR *bhir);
+    void *getMemoryPointer(bh_base &base, bool copy2host, bool force_alloc, bool nullify);
+    void setMemoryPointer(bh_base *base, bool host_ptr, void *mem);
+    void copyAllToHost();
+    void copyToHost(bh_base *base);
+    std::string info() const;
+    std::string statistics() const;
+    void resetStatistics() { stats = EngineStats(); }
+
+    /* Last flush's launches, for cost-model training and tests: (strategy, work items, device ms). */
+    struct LaunchRecord {
+        std::string strategy;
+        uint64_t work_items;
+        uint64_t grid;
+        double device_ms;
+        std::string kernel_hash;
+    };
+    std::vector<LaunchRecord> last_launches;
+
+    jitk::FusionConfig fusion_config;
+    bool profile = false;
+    /* Compile-only mode: plan and hiprtc-compile every kernel, touch no device. */
+    bool dry_run = false;
+    std::vector<std::string> dry_run_log;  // "strategy grid block hash" per planned launch
+
+private:
+    component::ComponentVE &comp;
+    WriterConfig wcfg;
+    std::string cache_dir;
+    bool resident = true;
+    std::unordered_map<const bh_base *, DeviceBuffer> bufs;
+    DevicePool pool;
+    unsigned char *reduce_scratch = nullptr;
+    uint64_t reduce_scratch_bytes = 0;
+    unsigned char *scan_scratch = nullptr;
+    uint64_t scan_scratch_bytes = 0;
+    unsigned int *counters = nullptr;  // [0]: reduce ticket, [1]: error flag
+    EngineStats stats;
+
+    DeviceBuffer &buffer(bh_base *base);
+    void ensureResident(bh_base *base);
+    void release(bh_base *base);
+    void ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need, bool zero);
+    void launch(const KernelPlan &plan);
+    void checkErrors();
+};
+
+} // namespace vehip
+} // namespace bohrium

@@ -1,0 +1,83 @@
+"""The benchmark programs, written against the bh107-style front end.
+
+They are ordinary array programs; the engine sees only their bytecode.
+    mc_pi          Monte Carlo pi: philox uniforms x, y; count(x*x + y*y <= 1) as int64
+    black_scholes  fused option pricing with the Abramowitz-Stegun CND polynomial, one sum per step
+    jacobi         5-point heat stencil on offset views with a max-abs-delta per iteration
+"""
+import numpy as np
+
+from . import bharray as _b
+from .bharray import BhArray, ufunc
+from .random import RandomState
+
+# Abramowitz & Stegun 26.2.17 coefficients of the normal CDF polynomial.
+CND_A = (0.31938153, -0.356563782, 1.781477937, -1.821255978, 1.330274429)
+CND_P = 0.2316419
+RSQRT2PI = 0.39894228040143267794
+
+
+def mc_pi(n, seed=42):
+    """x, y = philox uniforms (counters 0..n-1 and n..2n-1, key `seed`); returns the int64 count."""
+    R = RandomState(seed)
+    x = R.random_sample(n)
+    y = R.random_sample(n)
+    return ((x * x + y * y) <= 1.0).astype(np.int64).sum()
+
+
+def black_scholes_inputs(n, seed=42):
+    R = RandomState(seed)
+    S = R.uniform(58.0, 62.0, n)
+    K = R.uniform(55.0, 65.0, n)
+    T = R.uniform(0.25, 2.0, n)
+    return S, K, T
+
+
+def _cnd(d):
+    L = ufunc("absolute")(d)
+    k = 1.0 / (L * CND_P + 1.0)
+    a1, a2, a3, a4, a5 = CND_A
+    poly = k * (a1 + k * (a2 + k * (a3 + k * (a4 + k * a5))))
+    w = 1.0 - RSQRT2PI * ufunc("exp")(L * L * -0.5) * poly
+    neg = d < 0.0
+    return w + neg.astype(np.float64) * (1.0 - 2.0 * w)
+
+
+def black_scholes_step(S, K, T, step, r=0.02, sigma=0.3, dt=0.01):
+    """Sum of European call prices at time-to-expiry T + step*dt (one fused map + reduce)."""
+    Tt = T + step * dt
+    sq = ufunc("sqrt")(Tt)
+    d1 = (ufunc("log")(S / K) + (r + 0.5 * sigma * sigma) * Tt) / (sigma * sq)
+    d2 = d1 - sigma * sq
+    call = S * _cnd(d1) - K * ufunc("exp")(Tt * -r) * _cnd(d2)
+    return call.sum()
+
+
+def black_scholes(n, steps=10, seed=42):
+    S, K, T = black_scholes_inputs(n, seed)
+    return [black_scholes_step(S, K, T, i) for i in range(steps)]
+
+
+def jacobi_grid(n, dtype=np.float32):
+    """n x n grid: top row 40.0, the other borders -273.0, interior 0."""
+    G = _b.zeros((n, n), dtype)
+    G[:, 0] = -273.0
+    G[:, -1] = -273.0
+    G[-1, :] = -273.0
+    G[0, :] = 40.0
+    return G
+
+
+def jacobi_step(G):
+    """One Jacobi sweep in place; returns the max |new - old| of the interior (0-d array)."""
+    c = G[1:-1, 1:-1]
+    tmp = (c + G[:-2, 1:-1] + G[2:, 1:-1] + G[1:-1, :-2] + G[1:-1, 2:]) * 0.2
+    delta = ufunc("absolute")(tmp - c).max()
+    c[...] = tmp
+    return delta
+
+
+def jacobi(n, iters):
+    G = jacobi_grid(n)
+    deltas = [jacobi_step(G) for _ in range(iters)]
+    return G, deltas

@@ -1,0 +1,434 @@
+"""ORACLE — test infrastructure only.  Nothing in the product imports this.
+
+A CPU restatement of the reference OpenMP vector engine's semantics, used as
+the parity checker for the hip engine and as the CPU baseline in bench.py.
+
+It replays a bridge trace (bohrium_amd._bh_api.trace_start/stop) instruction
+by instruction.  Each instruction becomes a C99 loop nest whose scalar body is
+the reference's per-opcode C expression (core/jitk/instruction.cpp:58-578,
+special forms of core/jitk/engines/engine.cpp:251-336), compiled like the
+reference compiles its kernels (ve/openmp/CMakeLists.txt:58-93:
+`-x c -fPIC -shared -std=gnu99 -O3 -march=native -fopenmp`, with <tgmath.h>
+as in ve/openmp/engine_openmp.cpp:331-346) and loaded with ctypes.
+
+Semantics pinned here (and checked against NumPy / closed forms in tests):
+  * views: element address = start + sum(i_d * stride_d)            (view.cpp:31-88)
+  * reductions start from sweep_identity (instruction.cpp:546-578), including
+    its quirks: max-reduce of floats starts at the smallest positive normal,
+    bool min/max start at true, bitwise-and starts at the 32-bit all-ones value;
+    then accumulate sequentially along the axis
+  * accumulate: out[k] = out[k-1] OP in[k] with out[0] = identity OP in[0] (engine.cpp:299-308)
+  * random: out = random123(start, key, memory offset of the output element) (engine.cpp:263-277)
+  * range: out = memory offset of the output element                        (engine.cpp:254-262)
+  * gather: out[i] = in.base[in.start + idx[i]]; scatter: out.base[out.start + idx[i]] = in[i]
+Parity is asserted on results, not on fusion partitions (this oracle does not fuse).
+"""
+import ctypes
+import hashlib
+import math
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+
+# Reference compile flags of the OpenMP engine (ve/openmp/CMakeLists.txt:58-93); -w instead of -Werror
+# because the generated checker code may carry unused variables.
+CFLAGS = ["-x", "c", "-fPIC", "-shared", "-std=gnu99", "-O3", "-march=native", "-fopenmp", "-w"]
+
+# opcode ids (reference core/codegen/opcodes.json)
+OP = dict(ADD=1, SUBTRACT=2, MULTIPLY=3, DIVIDE=4, POWER=5, ABSOLUTE=6, GREATER=7, GREATER_EQUAL=8, LESS=9,
+          LESS_EQUAL=10, EQUAL=11, NOT_EQUAL=12, LOGICAL_AND=13, LOGICAL_OR=14, LOGICAL_XOR=15, LOGICAL_NOT=16,
+          MAXIMUM=17, MINIMUM=18, BITWISE_AND=19, BITWISE_OR=20, BITWISE_XOR=21, INVERT=22, LEFT_SHIFT=23,
+          RIGHT_SHIFT=24, COS=25, SIN=26, TAN=27, COSH=28, SINH=29, TANH=30, ARCSIN=31, ARCCOS=32, ARCTAN=33,
+          ARCSINH=34, ARCCOSH=35, ARCTANH=36, ARCTAN2=37, EXP=38, EXP2=39, EXPM1=40, LOG=41, LOG2=42, LOG10=43,
+          LOG1P=44, SQRT=45, CEIL=46, TRUNC=47, FLOOR=48, RINT=49, MOD=50, ISNAN=51, ISINF=52, IDENTITY=53,
+          FREE=55, NONE=57, TALLY=58, ADD_REDUCE=59, MULTIPLY_REDUCE=60, MINIMUM_REDUCE=61, MAXIMUM_REDUCE=62,
+          LOGICAL_AND_REDUCE=63, BITWISE_AND_REDUCE=64, LOGICAL_OR_REDUCE=65, BITWISE_OR_REDUCE=66,
+          LOGICAL_XOR_REDUCE=67, BITWISE_XOR_REDUCE=68, RANDOM=71, RANGE=72, REAL=73, IMAG=74,
+          ADD_ACCUMULATE=75, MULTIPLY_ACCUMULATE=76, SIGN=77, GATHER=79, SCATTER=80, REMAINDER=81,
+          COND_SCATTER=82, ISFINITE=83, CONJ=84)
+NAME = {v: k for k, v in OP.items()}
+REDUCE = {59, 60, 61, 62, 63, 64, 65, 66, 67, 68}
+ACCUMULATE = {75, 76}
+
+CTYPE = {"|b1": "bool", "|i1": "int8_t", "<i2": "int16_t", "<i4": "int32_t", "<i8": "int64_t", "|u1": "uint8_t",
+         "<u2": "uint16_t", "<u4": "uint32_t", "<u8": "uint64_t", "<f4": "float", "<f8": "double",
+         "<c8": "float complex", "<c16": "double complex"}
+
+
+def _kind(dt):
+    return np.dtype(dt).kind
+
+
+def c_literal(dt, value):
+    """An exact C99 literal of numpy dtype `dt`."""
+    dt = np.dtype(dt)
+    v = np.asarray(value).astype(dt)[()]
+    if dt.kind == "b":
+        return "((bool)%d)" % int(bool(v))
+    if dt.kind in "iu":
+        iv = int(v)
+        if dt == np.int64 and iv == -2 ** 63:
+            return "((int64_t)(-9223372036854775807LL - 1))"
+        suffix = "ULL" if dt.kind == "u" else "LL"
+        return "((%s)%d%s)" % (CTYPE[dt.str], iv, suffix)
+    if dt.kind == "f":
+        return _c_float(float(v), dt == np.float32)
+    if dt.kind == "c":
+        single = dt == np.complex64
+        return "((%s)(%s + %s * I))" % (CTYPE[dt.str], _c_float(float(v.real), single), _c_float(float(v.imag), single))
+    raise ValueError(dt)
+
+
+def _c_float(x, single):
+    if math.isnan(x):
+        return "NAN"
+    if math.isinf(x):
+        return "(-INFINITY)" if x < 0 else "INFINITY"
+    return "(%s%s)" % (float.hex(x), "f" if single else "")
+
+
+def sweep_identity(opcode, dt):
+    """Identity of a sweep (reference core/jitk/instruction.cpp:546-578 and core/bh_constant.cpp:32-104)."""
+    dt = np.dtype(dt)
+    if opcode in (OP["ADD_REDUCE"], OP["BITWISE_OR_REDUCE"], OP["BITWISE_XOR_REDUCE"], OP["LOGICAL_OR_REDUCE"],
+                  OP["LOGICAL_XOR_REDUCE"], OP["ADD_ACCUMULATE"]):
+        return np.zeros((), dt)[()]
+    if opcode in (OP["MULTIPLY_REDUCE"], OP["MULTIPLY_ACCUMULATE"]):
+        return np.ones((), dt)[()]
+    if opcode in (OP["BITWISE_AND_REDUCE"], OP["LOGICAL_AND_REDUCE"]):
+        # bh_constant(~0u, dtype): the unsigned 32-bit all-ones value converted with a C cast.
+        return np.array(0xFFFFFFFF, np.uint64).astype(dt)[()] if dt.kind != "b" else np.bool_(True)
+    if opcode == OP["MAXIMUM_REDUCE"]:
+        if dt.kind == "b":
+            return np.bool_(True)
+        if dt.kind == "f":
+            return np.finfo(dt).tiny  # numeric_limits<T>::min(): smallest positive normal
+        if dt.kind == "i":
+            return np.array(np.iinfo(dt).min + 1, dt)[()]
+        return np.zeros((), dt)[()]
+    if opcode == OP["MINIMUM_REDUCE"]:
+        if dt.kind == "b":
+            return np.bool_(True)
+        if dt.kind == "f":
+            return np.finfo(dt).max
+        return np.array(np.iinfo(dt).max, dt)[()]
+    raise ValueError("no identity for opcode %d" % opcode)
+
+
+def c_operation(opcode, types, ops):
+    """C99 statement for one element (reference write_operation, non-OpenCL branch).
+    `types` are the numpy dtypes of the operands (constant included), `ops` their C spellings."""
+    a = ops
+    t0 = np.dtype(types[0])
+    t1 = np.dtype(types[1]) if len(types) > 1 else None
+
+    def un(fn):
+        return "%s = %s(%s);" % (a[0], fn, a[1])
+
+    def bin_(op):
+        return "%s = %s %s %s;" % (a[0], a[1], op, a[2])
+
+    n = NAME[opcode]
+    simple = {"BITWISE_AND": "&", "BITWISE_OR": "|", "BITWISE_XOR": "^", "LOGICAL_OR": "||", "LOGICAL_AND": "&&",
+              "LEFT_SHIFT": "<<", "RIGHT_SHIFT": ">>", "GREATER": ">", "GREATER_EQUAL": ">=", "LESS": "<",
+              "LESS_EQUAL": "<=", "NOT_EQUAL": "!=", "EQUAL": "==", "ADD": "+", "SUBTRACT": "-", "MULTIPLY": "*",
+              "ADD_ACCUMULATE": "+", "MULTIPLY_ACCUMULATE": "*"}
+    if n in simple:
+        return bin_(simple[n])
+    if n == "BITWISE_AND_REDUCE":
+        return "%s = %s & %s;" % (a[0], a[0], a[1])
+    if n == "BITWISE_OR_REDUCE":
+        return "%s = %s | %s;" % (a[0], a[0], a[1])
+    if n == "BITWISE_XOR_REDUCE":
+        return "%s = %s ^ %s;" % (a[0], a[0], a[1])
+    if n == "LOGICAL_NOT":
+        return "%s = !%s;" % (a[0], a[1])
+    if n == "LOGICAL_OR_REDUCE":
+        return "%s = %s || %s;" % (a[0], a[0], a[1])
+    if n == "LOGICAL_AND_REDUCE":
+        return "%s = %s && %s;" % (a[0], a[0], a[1])
+    if n == "LOGICAL_XOR":
+        return "%s = !%s != !%s;" % (a[0], a[1], a[2])
+    if n == "LOGICAL_XOR_REDUCE":
+        return "%s = !%s != !%s;" % (a[0], a[0], a[1])
+    if n == "MAXIMUM":
+        return "%s = %s > %s ? %s : %s;" % (a[0], a[1], a[2], a[1], a[2])
+    if n == "MAXIMUM_REDUCE":
+        return "%s = %s > %s ? %s : %s;" % (a[0], a[0], a[1], a[0], a[1])
+    if n == "MINIMUM":
+        return "%s = %s < %s ? %s : %s;" % (a[0], a[1], a[2], a[1], a[2])
+    if n == "MINIMUM_REDUCE":
+        return "%s = %s < %s ? %s : %s;" % (a[0], a[0], a[1], a[0], a[1])
+    if n == "INVERT":
+        return ("%s = !%s;" if t0.kind == "b" else "%s = ~%s;") % (a[0], a[1])
+    if n == "MOD":
+        if t0.kind == "f":
+            return "%s = fmod(%s, %s);" % (a[0], a[1], a[2])
+        return bin_("%")
+    if n == "REMAINDER":
+        if t0.kind == "f":
+            return "%s = %s - floor(%s / %s) * %s;" % (a[0], a[1], a[1], a[2], a[2])
+        if t0.kind == "u":
+            return bin_("%")
+        return ("%s = ((%s > 0) == (%s > 0) || (%s %% %s) == 0) ? (%s %% %s) : (%s %% %s) + %s;"
+                % (a[0], a[1], a[2], a[1], a[2], a[1], a[2], a[1], a[2], a[2]))
+    fns = {"RINT": "rint", "EXP2": "exp2", "EXPM1": "expm1", "LOG1P": "log1p", "ARCSIN": "asin", "ARCCOS": "acos",
+           "ARCTAN": "atan", "ARCSINH": "asinh", "ARCCOSH": "acosh", "ARCTANH": "atanh", "FLOOR": "floor",
+           "CEIL": "ceil", "TRUNC": "trunc", "LOG2": "log2", "SIN": "sin", "COS": "cos", "TAN": "tan",
+           "SINH": "sinh", "COSH": "cosh", "TANH": "tanh", "EXP": "exp", "SQRT": "sqrt", "LOG": "log",
+           "CONJ": "conj", "REAL": "creal", "IMAG": "cimag"}
+    if n in fns:
+        return un(fns[n])
+    if n == "ARCTAN2":
+        return "%s = atan2(%s, %s);" % (a[0], a[1], a[2])
+    if n in ("ISNAN", "ISINF", "ISFINITE"):
+        fn = n.lower()
+        if t1.kind == "c":
+            return "%s = %s(creal(%s));" % (a[0], fn, a[1])
+        if t1.kind == "f":
+            return "%s = %s(%s);" % (a[0], fn, a[1])
+        return "%s = %s;" % (a[0], "true" if n == "ISFINITE" else "false")
+    if n in ("RANGE", "RANDOM", "GATHER", "SCATTER"):
+        return "%s = %s;" % (a[0], a[1])
+    if n == "COND_SCATTER":
+        return "if (%s) { %s = %s; }" % (a[2], a[0], a[1])
+    if n == "ABSOLUTE":
+        if t1.kind == "b" or t1.kind == "u":
+            return "%s = %s;" % (a[0], a[1])
+        if t1.kind == "f":
+            return un("fabs")
+        if t1 == np.int64:
+            return un("llabs")
+        return "%s = abs((int)%s);" % (a[0], a[1])
+    if n == "POWER":
+        if t0.kind in "iub":
+            return "%s = pow((double) %s, (double)%s);" % (a[0], a[1], a[2])
+        return "%s = pow(%s, %s);" % (a[0], a[1], a[2])
+    if n == "ADD_REDUCE":
+        return "%s += %s;" % (a[0], a[1])
+    if n == "MULTIPLY_REDUCE":
+        return "%s *= %s;" % (a[0], a[1])
+    if n == "DIVIDE":
+        if t0.kind == "i":
+            return ("%s = ((%s > 0) != (%s > 0) && (%s %% %s) != 0)?(%s / %s - 1):(%s / %s);"
+                    % (a[0], a[1], a[2], a[1], a[2], a[1], a[2], a[1], a[2]))
+        return bin_("/")
+    if n == "IDENTITY":
+        return "%s = %s;" % (a[0], a[1])
+    if n == "LOG10":
+        if t0.kind == "c":
+            return "%s = clog(%s) / log(10.0f);" % (a[0], a[1])
+        return un("log10")
+    if n == "SIGN":
+        if t0.kind == "c":
+            ct = "float" if t0 == np.complex64 else "double"
+            return ("{ %s real = creal(%s); %s imag = cimag(%s); %s = (real == 0 ? ((imag > 0) - (0 > imag)) : "
+                    "((real > 0) - (0 > real))); }" % (ct, a[1], ct, a[1], a[0]))
+        return "%s = ((%s > 0) - (0 > %s));" % (a[0], a[1], a[1])
+    raise NotImplementedError("oracle: opcode %s" % n)
+
+
+def _index(view, idx_names):
+    """start + sum(i_d * stride_d) with literal numbers."""
+    start, shape, stride = view[1], view[2], view[3]
+    terms = [str(start)]
+    for name, st in zip(idx_names, stride):
+        if st != 0:
+            terms.append("%s*(%d)" % (name, st))
+    return "(" + " + ".join(terms) + ")"
+
+
+class _Gen:
+    def __init__(self, dtypes):
+        self.dtypes = dtypes  # bid -> dtype str
+        self.lines = []
+
+    def arr(self, bid):
+        return "((%s*)d[%d])" % (CTYPE[self.dtypes[bid]], bid)
+
+    def ref(self, view, idx_names):
+        return "%s[%s]" % (self.arr(view[0]), _index(view, idx_names))
+
+    def loops(self, shape, body, names=None, indent="    "):
+        names = names or ["i%d" % k for k in range(len(shape))]
+        out = []
+        for k, (n, s) in enumerate(zip(names, shape)):
+            out.append("%sfor (int64_t %s = 0; %s < %d; ++%s) {" % (indent + "  " * k, n, n, s, n))
+        out.extend(indent + "  " * len(shape) + line for line in body)
+        for k in reversed(range(len(shape))):
+            out.append(indent + "  " * k + "}")
+        return out
+
+    def instr(self, opcode, ops, const):
+        L = []
+        if opcode in (OP["FREE"], OP["NONE"], OP["TALLY"]):
+            return L
+        out = ops[0]
+        types = []
+        for o in ops:
+            if o is None:
+                types.append(np.dtype(const[0]) if const[0] != "r123" else np.dtype(np.uint64))
+            else:
+                types.append(np.dtype(self.dtypes[o[0]]))
+        if opcode in REDUCE or opcode in ACCUMULATE:
+            inp, axis = ops[1], int(const[1])
+            shape = inp[2]
+            t = np.dtype(self.dtypes[out[0]])
+            ident = c_literal(t, sweep_identity(opcode, t))
+            names = ["i%d" % k for k in range(len(shape))]
+            outer = [n for k, n in enumerate(names) if k != axis]
+            outer_shape = [s for k, s in enumerate(shape) if k != axis]
+            in_ref = self.ref(inp, names)
+            body = ["%s acc = %s;" % (CTYPE[t.str], ident)]
+            inner = []
+            if opcode in REDUCE:
+                inner.append(c_operation(opcode, [t, t], ["acc", in_ref]))
+                out_names = outer if len(shape) > 1 else []
+                out_ref = self.ref(out, out_names) if out_names else "%s[%d]" % (self.arr(out[0]), out[1])
+                body += self.loops([shape[axis]], inner, [names[axis]], indent="")
+                body.append("%s = acc;" % out_ref)
+            else:
+                inner.append(c_operation(opcode, [t, t, t], ["acc", "acc", in_ref]))
+                inner.append("%s = acc;" % self.ref(out, names))
+                body += self.loops([shape[axis]], inner, [names[axis]], indent="")
+            L += ["  {"] + self.loops(outer_shape, body, outer) + ["  }"]
+            return L
+        shape = out[2]
+        if opcode in (OP["GATHER"], OP["SCATTER"], OP["COND_SCATTER"]):
+            shape = ops[2][2]
+        names = ["i%d" % k for k in range(len(shape))]
+        if opcode == OP["RANDOM"]:
+            start, key = const[1]
+            body = ["%s = oracle_random123(%dULL, %dULL, (uint64_t)%s);" % (
+                self.ref(out, names), start, key, _index(out, names))]
+        elif opcode == OP["RANGE"]:
+            body = ["%s = %s;" % (self.ref(out, names), _index(out, names))]
+        elif opcode == OP["GATHER"]:
+            inp, idx = ops[1], ops[2]
+            nb = self.nelem[inp[0]]
+            body = ["{ int64_t g = %d + (int64_t)%s;" % (inp[1], self.ref(idx, names)),
+                    "  if (g < 0 || g >= %d) err = 1; else %s = %s[g]; }" % (nb, self.ref(out, names),
+                                                                          self.arr(inp[0]))]
+        elif opcode in (OP["SCATTER"], OP["COND_SCATTER"]):
+            val, idx = ops[1], ops[2]
+            nb = self.nelem[out[0]]
+            cond = "" if opcode == OP["SCATTER"] else "if (%s) " % self.ref(ops[3], names)
+            body = ["%s{ int64_t g = %d + (int64_t)%s;" % (cond, out[1], self.ref(idx, names)),
+                    "  if (g < 0 || g >= %d) err = 1; else %s[g] = %s; }" % (nb, self.arr(out[0]),
+                                                                          self.ref(val, names))]
+        else:
+            cops = []
+            for o in ops:
+                if o is None:
+                    cops.append(c_literal(const[0], const[1]))
+                else:
+                    cops.append(self.ref(o, names))
+            body = [c_operation(opcode, types, cops)]
+        return ["  {"] + self.loops(shape, body) + ["  }"]
+
+
+def _compile(source):
+    h = hashlib.sha1(source.encode()).hexdigest()[:20]
+    os.makedirs(BUILD, exist_ok=True)
+    so = os.path.join(BUILD, "oracle_%s.so" % h)
+    if not os.path.exists(so):
+        with tempfile.NamedTemporaryFile("w", suffix=".c", dir=BUILD, delete=False) as f:
+            f.write(source)
+            src = f.name
+        tmp = so + ".tmp%d" % os.getpid()
+        r = subprocess.run(["gcc"] + CFLAGS + ["-I" + CSRC, src, "-o", tmp, "-lm"],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        os.unlink(src)
+        if r.returncode != 0:
+            raise RuntimeError("oracle: gcc failed:\n" + r.stdout[-4000:])
+        os.replace(tmp, so)
+    return ctypes.CDLL(so)
+
+
+def build_checker():
+    """Compile the checker's fixed C piece (the philox restatement) once, as a build check."""
+    src = HEADER + "\nuint64_t oracle_philox(uint64_t s, uint64_t k, uint64_t i) { return oracle_random123(s, k, i); }\n"
+    lib = _compile(src)
+    lib.oracle_philox.restype = ctypes.c_uint64
+    lib.oracle_philox.argtypes = [ctypes.c_uint64] * 3
+    return lib
+
+
+HEADER = """#include <stdint.h>
+#include <stdlib.h>
+#include <stdbool.h>
+#include <complex.h>
+#include <tgmath.h>
+#include <math.h>
+#include "r123_philox.h"
+"""
+
+
+def _split_groups(trace):
+    """Split a trace into flush groups: (records, nrepeats, cond_bid)."""
+    groups, cur = [], []
+    for rec in trace:
+        if rec[0] == "flush":
+            groups.append((cur, rec[1], rec[2]))
+            cur = []
+        else:
+            cur.append(rec)
+    if cur:
+        groups.append((cur, 1, None))
+    return groups
+
+
+def replay(trace, keep=None):
+    """Execute `trace` on the CPU.  Returns {base id: numpy array} for every base
+    still alive at the end (or only those in `keep`)."""
+    dtypes, nelem = {}, {}
+    for rec in trace:
+        if rec[0] == "new":
+            dtypes[rec[1]] = np.dtype(rec[2]).str
+            nelem[rec[1]] = rec[3]
+    gen = _Gen(dtypes)
+    gen.nelem = nelem
+    groups = _split_groups(trace)
+    src = [HEADER]
+    for gi, (recs, _, _) in enumerate(groups):
+        src.append("int g%d(void **d) {\n  int err = 0;" % gi)
+        for rec in recs:
+            if rec[0] == "op":
+                src.extend(gen.instr(rec[1], rec[2], rec[3]))
+        src.append("  return err;\n}\n")
+    lib = _compile("\n".join(src))
+    data = {}
+    freed = set()
+    maxbid = max(dtypes) + 1 if dtypes else 1
+    for gi, (recs, nrep, cond) in enumerate(groups):
+        for rec in recs:
+            if rec[0] == "new":
+                data[rec[1]] = np.zeros(max(1, rec[3]), dtype=np.dtype(rec[2]))
+            elif rec[0] == "data":
+                data[rec[1]] = np.array(rec[2], dtype=np.dtype(dtypes[rec[1]]), copy=True).reshape(-1)
+            elif rec[0] == "free":
+                freed.add(rec[1])
+            elif rec[0] == "ext":
+                raise NotImplementedError("oracle: extension method %s" % rec[1])
+        ptrs = (ctypes.c_void_p * maxbid)()
+        for b, arr in data.items():
+            ptrs[b] = arr.ctypes.data
+        fn = getattr(lib, "g%d" % gi)
+        fn.argtypes = [ctypes.c_void_p * maxbid]
+        for _ in range(int(nrep)):
+            if fn(ptrs) != 0:
+                raise IndexError("oracle: gather/scatter index out of bounds")
+            if cond is not None and not bool(data[cond][0]):
+                break
+        for b in list(freed):
+            data.pop(b, None)
+        freed.clear()
+    if keep is not None:
+        return {b: data[b] for b in keep if b in data}
+    return data

@@ -1,0 +1,76 @@
+"""The hip engine's fuser and code generator on the CPU: run programs with the
+engine in compile-only mode (BH_HIP_DRY_RUN=1, every kernel is planned and
+compiled by hiprtc for gfx950, nothing is launched) and check the plans."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+DRIVER = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, %(root)r)
+import bohrium_amd as bh
+from bohrium_amd import workloads
+plans = {}
+def plan(name, fn):
+    fn()
+    bh.flush()
+    plans[name] = [l.split()[0] for l in bh.message("plan log").splitlines() if l.strip()]
+plan("mc_pi", lambda: workloads.mc_pi(10 ** 6))
+S, K, T = workloads.black_scholes_inputs(1000)
+bh.flush(); bh.message("plan log")
+plan("bs_step", lambda: workloads.black_scholes_step(S, K, T, 3))
+G = workloads.jacobi_grid(64)
+bh.flush(); bh.message("plan log")
+plan("jacobi_step", lambda: workloads.jacobi_step(G))
+plan("scan", lambda: (bh.arange(1000) > 5).astype(np.int64).cumsum())
+plan("row_sum", lambda: bh.ones((100, 50)).sum(axis=1))
+plan("col_sum", lambda: bh.ones((100, 50)).sum(axis=0))
+plan("cumsum_axis0", lambda: bh.ones((100, 50)).cumsum(axis=0))
+plan("gather", lambda: bh.take(bh.arange(50).astype(np.float32), bh.array(np.arange(10) * 3)))
+print("PLANS=" + json.dumps(plans))
+"""
+
+
+@pytest.fixture(scope="module")
+def plans():
+    env = dict(os.environ, BH_HIP_DRY_RUN="1")
+    r = subprocess.run([sys.executable, "-c", DRIVER % {"root": ROOT}], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:]
+    line = [l for l in r.stdout.splitlines() if l.startswith("PLANS=")][0]
+    return json.loads(line[6:])
+
+
+def test_mc_pi_is_one_fused_reduction(plans):
+    """philox x2, casts, x*x+y*y, compare and the int64 sum run as one launch."""
+    assert plans["mc_pi"] == ["reduce"]
+
+
+def test_black_scholes_step_is_one_fused_reduction(plans):
+    assert plans["bs_step"] == ["reduce"]
+
+
+def test_jacobi_step_plan(plans):
+    # stencil + |tmp - c| (2-d map), the max over the flattened delta, the copy back
+    assert plans["jacobi_step"].count("serial") == 0
+    assert "reduce" in plans["jacobi_step"]
+
+
+def test_scan_plan(plans):
+    assert plans["scan"] == ["scan"]
+
+
+def test_axis_reductions_run_on_parallel_skeletons(plans):
+    assert plans["row_sum"] == ["map"] or plans["row_sum"] == ["map", "map"]
+    assert "serial" not in plans["col_sum"]
+    assert "serial" not in plans["cumsum_axis0"]
+
+
+def test_gather_plan(plans):
+    assert "serial" not in plans["gather"]

@@ -1,0 +1,265 @@
+"""Parity of the hip engine with the oracle on the same bridge trace (MI355X).
+
+Each test runs a program on the engine with the trace recorded, replays the
+identical trace on the oracle and compares: bit-exact for integer/bool/index/
+philox results; fp64 within 1e-12 and fp32 within 1e-5 relative (plus the
+stated absolute slack where a reduction order differs)."""
+import numpy as np
+import pytest
+
+import bohrium_amd as bh
+from bohrium_amd import workloads
+from tests.helpers import run_both, assert_parity
+
+pytestmark = pytest.mark.gpu
+
+INT_TYPES = [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16, np.uint32, np.uint64]
+FLOAT_TYPES = [np.float32, np.float64]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def engine_is_hip():
+    info = bh.message("info")
+    assert "HIP vector engine" in info, "the hip engine is not the active component: %r" % info
+    yield
+
+
+def _inputs(dtype, n=1000, seed=0, positive=False, nonzero=False):
+    rng = np.random.RandomState(seed)
+    dtype = np.dtype(dtype)
+    if dtype.kind == "b":
+        return rng.randint(0, 2, n).astype(bool)
+    if dtype.kind in "iu":
+        lo = 1 if (positive or nonzero or dtype.kind == "u") else -50
+        a = rng.randint(lo, 50, n).astype(dtype)
+        if nonzero:
+            a[a == 0] = 1
+        return a
+    if dtype.kind == "c":
+        return (rng.uniform(-2, 2, n) + 1j * rng.uniform(-2, 2, n)).astype(dtype)
+    return rng.uniform(0.1 if positive else -2.0, 2.0, n).astype(dtype)
+
+
+@pytest.mark.parametrize("name", ["add", "subtract", "multiply", "maximum", "minimum", "greater", "less_equal",
+                                  "equal", "not_equal"])
+@pytest.mark.parametrize("dtype", INT_TYPES + FLOAT_TYPES)
+def test_binary(name, dtype):
+    x, y = _inputs(dtype, seed=1), _inputs(dtype, seed=2)
+    got, want = run_both(lambda: bh.ufunc(name)(bh.array(x), bh.array(y)))
+    assert_parity(got[0], want[0], got[0].dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.int32, np.int64, np.uint16, np.uint64])
+def test_integer_division_remainder_bitwise(dtype):
+    x, y = _inputs(dtype, seed=3), _inputs(dtype, seed=4, nonzero=True)
+    prog = lambda: [bh.array(x) // bh.array(y), bh.ufunc("remainder")(bh.array(x), bh.array(y)),
+                    bh.ufunc("fmod")(bh.array(x), bh.array(y)), bh.array(x) & bh.array(y),
+                    bh.array(x) ^ bh.array(y), bh.ufunc("left_shift")(bh.array(x), bh.array(y) % 3),
+                    bh.ufunc("invert")(bh.array(x)), bh.ufunc("absolute")(bh.array(x)), bh.array(x) ** 2]
+    got, want = run_both(prog)
+    for g, w in zip(got, want):
+        assert_parity(g, w, g.dtype)
+
+
+UNARY = ["sqrt", "exp", "log", "sin", "cos", "tan", "tanh", "sinh", "cosh", "arctan", "arcsin", "arccos",
+         "exp2", "expm1", "log2", "log10", "log1p", "floor", "ceil", "trunc", "rint", "absolute", "sign",
+         "arcsinh", "isnan", "isinf", "isfinite"]
+
+
+@pytest.mark.parametrize("dtype", FLOAT_TYPES)
+def test_unary_float(dtype):
+    x = _inputs(dtype, seed=5, positive=True) * np.asarray(0.45, dtype)
+    got, want = run_both(lambda: [bh.ufunc(n)(bh.array(x)) for n in UNARY])
+    for n, g, w in zip(UNARY, got, want):
+        assert_parity(g, w, g.dtype, rtol=2e-6 if dtype == np.float32 else 1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("src", [np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_])
+def test_identity_casts(src):
+    x = _inputs(src, seed=6, positive=True)
+    dsts = [np.float64, np.float32, np.int64, np.int16, np.uint32, np.bool_]
+    got, want = run_both(lambda: [bh.array(x).astype(d) for d in dsts])
+    for g, w in zip(got, want):
+        assert_parity(g, w, g.dtype)
+
+
+def test_complex():
+    x, y = _inputs(np.complex128, seed=7), _inputs(np.complex128, seed=8)
+    prog = lambda: [bh.array(x) + bh.array(y), bh.array(x) * bh.array(y), bh.array(x) / bh.array(y),
+                    bh.ufunc("exp")(bh.array(x)), bh.ufunc("real")(bh.array(x)), bh.ufunc("sqrt")(bh.array(x))]
+    got, want = run_both(prog)
+    for g, w in zip(got, want):
+        assert_parity(g, w, g.dtype, rtol=1e-12)
+
+
+def _random_view(shape, seed):
+    rng = np.random.RandomState(seed)
+    key = []
+    for n in shape:
+        start = rng.randint(0, max(1, n // 3))
+        step = int(rng.choice([1, 2, -1]))
+        key.append(slice(start, n, step) if step > 0 else slice(n - 1 - start, None, step))
+    return tuple(key)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("ndim", [1, 2, 3, 4])
+def test_reduce_views_every_axis(seed, ndim):
+    shape = (9, 7, 5, 6)[:ndim]
+    x = np.random.RandomState(seed).uniform(-1, 1, shape).astype(np.float32)
+    xi = (x * 100).astype(np.int64)
+    key = _random_view(shape, seed)
+
+    def prog():
+        out = []
+        for axis in range(ndim):
+            out.append(bh.array(x)[key].sum(axis=axis))
+            out.append(bh.ufunc("maximum").reduce(bh.array(xi)[key], axis))
+            out.append(bh.ufunc("minimum").reduce(bh.array(x)[key], axis))
+        return out
+    got, want = run_both(prog)
+    for g, w in zip(got, want):
+        assert_parity(g, w, g.dtype, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("name", ["add", "multiply", "minimum", "maximum", "bitwise_or", "bitwise_xor",
+                                  "bitwise_and", "logical_or", "logical_and"])
+@pytest.mark.parametrize("n", [1, 10, 1000, 1 << 20])
+def test_full_reductions(name, n):
+    dts = [np.int64, np.uint32, np.float64] if not name.startswith("logical") else [np.bool_]
+    if name.startswith("bitwise"):
+        dts = [np.int64, np.uint32, np.int8]
+    for dt in dts:
+        x = _inputs(dt, n=n, seed=9, positive=True)
+        if name == "multiply" and np.dtype(dt).kind == "f":
+            x = (1.0 + (x - 1.0) * 1e-6).astype(dt)
+        got, want = run_both(lambda: bh.ufunc(name).reduce(bh.array(x)))
+        assert_parity(got[0], want[0], got[0].dtype, rtol=1e-9)
+
+
+def test_reduce_identity_quirks():
+    neg = -np.arange(1.0, 1001.0)
+    big = np.array([0x7FFFFFFF00000FFF, 0x0FFFFFFFFFFFFF0F] * 500, dtype=np.int64)
+    got, want = run_both(lambda: [bh.array(neg).max(), bh.ufunc("bitwise_and").reduce(bh.array(big))])
+    assert got[0] == want[0] == np.finfo(np.float64).tiny
+    assert got[1] == want[1]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_accumulate_views(axis):
+    x = np.random.RandomState(10).uniform(-1, 1, (30, 20, 10))
+    xi = (x * 10).astype(np.int64)
+    key = _random_view(x.shape, 4)
+    got, want = run_both(lambda: [bh.array(x)[key].cumsum(axis=axis), bh.array(xi)[key].cumprod(axis=axis)])
+    assert_parity(got[0], want[0], np.float64, rtol=1e-12, atol=1e-12)
+    assert_parity(got[1], want[1], np.int64)
+
+
+@pytest.mark.parametrize("n", [1, 100, 2048, 2049, 100000, 3000001])
+def test_scan_1d_int64_exact(n):
+    x = np.random.RandomState(11).randint(-3, 5, n).astype(np.int32)
+    got, want = run_both(lambda: (bh.array(x) > 0).astype(np.int64).cumsum())
+    assert_parity(got[0], want[0], np.int64)
+    np.testing.assert_array_equal(got[0], np.cumsum(x > 0))
+
+
+def test_scan_float_and_product():
+    x = np.random.RandomState(12).uniform(0.999, 1.001, 50000)
+    got, want = run_both(lambda: [bh.array(x).cumsum(), bh.array(x).cumprod()])
+    assert_parity(got[0], want[0], np.float64, rtol=1e-10)
+    assert_parity(got[1], want[1], np.float64, rtol=1e-10)
+
+
+def test_gather_scatter():
+    rng = np.random.RandomState(13)
+    data = rng.uniform(size=100000).astype(np.float32)
+    idx = rng.randint(0, 100000, 60000).astype(np.int64)
+    uidx = rng.permutation(100000)[:30000].astype(np.int64)
+    vals = rng.uniform(size=30000).astype(np.float32)
+    mask = rng.randint(0, 2, 30000).astype(bool)
+
+    def prog():
+        g = bh.take(bh.array(data), bh.array(idx))
+        a = bh.array(data)
+        bh.put(a, bh.array(uidx), bh.array(vals))
+        b = bh.array(data)
+        bh.put(b, bh.array(uidx), bh.array(vals), mask=bh.array(mask))
+        return [g, a, b]
+    got, want = run_both(prog)
+    for g, w in zip(got, want):
+        assert_parity(g, w, g.dtype)
+
+
+def test_gather_out_of_bounds_raises():
+    with pytest.raises(Exception):
+        bh.take(bh.arange(10), bh.array(np.array([3, 11], np.int64))).asnumpy()
+    # The engine stays usable afterwards.
+    np.testing.assert_array_equal(bh.arange(5).asnumpy(), np.arange(5))
+
+
+@pytest.mark.parametrize("n", [1, 1000, 1000003])
+def test_random_is_bit_exact(n):
+    got, want = run_both(lambda: [bh.random.RandomState(42).random123(n),
+                                  bh.random.RandomState(7).random_sample((n, 3))[:, 1]])
+    assert_parity(got[0], want[0], np.uint64)
+    # float64(u64) / 2**64 is exact on both sides, so the uniforms are bit-exact too.
+    np.testing.assert_array_equal(got[1], want[1])
+
+
+def test_mc_pi_1e6_bit_exact():
+    got, want = run_both(lambda: workloads.mc_pi(10 ** 6))
+    assert int(got[0]) == int(want[0])
+
+
+def test_mc_pi_full_config_bit_exact():
+    """BASELINE config: N = 1e7 each, seed 42; the count must be bit-exact."""
+    got, want = run_both(lambda: workloads.mc_pi(10 ** 7))
+    assert int(got[0]) == int(want[0])
+    assert abs(4.0 * int(got[0]) / 1e7 - np.pi) < 2e-3
+
+
+def test_black_scholes_small():
+    got, want = run_both(lambda: workloads.black_scholes(100003, steps=3))
+    for g, w in zip(got, want):
+        assert_parity(g, w, np.float64, rtol=1e-12)
+
+
+def test_jacobi_small():
+    def prog():
+        G, deltas = workloads.jacobi(130, 20)
+        return [G] + deltas
+    got, want = run_both(prog)
+    for g, w in zip(got, want):
+        assert_parity(g, w, np.float32, rtol=1e-5, atol=1e-3)
+
+
+def test_do_while_known_answers():
+    def prog():
+        a = bh.zeros(4)
+
+        def body(a):
+            a += 1
+        bh.do_while(body, 5, a)
+        b = bh.zeros(4)
+
+        def body2(b):
+            b += 1
+            return bh.sum(b) < 10
+        bh.do_while(body2, None, b)
+        return [a, b]
+    got, want = run_both(prog)
+    np.testing.assert_array_equal(got[0], [5, 5, 5, 5])
+    np.testing.assert_array_equal(got[1], [3, 3, 3, 3])
+    np.testing.assert_array_equal(want[1], [3, 3, 3, 3])
+
+
+def test_empty_and_ragged():
+    def prog():
+        e = bh.zeros(0)
+        s = e.sum()
+        r = bh.array(np.arange(7.0))[::3] * 2
+        one = bh.array(np.array([5.0])).sum()
+        return [s, r, one]
+    got, want = run_both(prog)
+    for g, w in zip(got, want):
+        assert_parity(g, w, np.float64)
