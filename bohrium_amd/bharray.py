@@ -376,6 +376,14 @@ def flush():
     api.flush()
 
 
+def asnumpy_many(arrays):
+    """Fetch several arrays with a single flush."""
+    for a in arrays:
+        api.sync(a.base.handle)
+    api.flush()
+    return [a.asnumpy() for a in arrays]
+
+
 def take(a, indices):
     """out[i] = a.flat[indices[i]] for a contiguous `a` (BH_GATHER)."""
     if not a.iscontiguous():

@@ -202,7 +202,7 @@ std::string hip_operation(const bh_instruction &instr, const std::vector<std::st
             if (bh_type_is_complex(instr.operand_type(0))) {
                 o << a[0] << " = bh_cpow(" << a[1] << ", " << a[2] << ");";
             } else if (bh_type_is_integer(instr.operand_type(0)) || instr.operand_type(0) == bh_type::BOOL) {
-                o << a[0] << " = pow((double) " << a[1] << ", (double)" << a[2] << ");";
+                o << a[0] << " = bh_pow_int((double) " << a[1] << ", (double)" << a[2] << ");";
             } else {
                 o << a[0] << " = pow(" << a[1] << ", " << a[2] << ");";
             }

@@ -265,16 +265,24 @@ void EngineHip::launch(const KernelPlan &plan) {
     }
     if (profile) {
         bhhip_event_record(ev1);
-        float ms = 0;
-        bhhip_event_elapsed_ms(ev0, ev1, &ms);
-        bhhip_event_destroy(ev0);
-        bhhip_event_destroy(ev1);
-        stats.device_ms_by_strategy[strategy_name(plan.strategy)] += ms;
         char hex[32];
         std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(fnv1a(plan.source)));
-        last_launches.push_back({strategy_name(plan.strategy), plan.work_items, plan.grid, ms, hex});
+        pending_events.push_back({ev0, ev1, {strategy_name(plan.strategy), plan.work_items, plan.grid, 0.0, hex}});
     }
     stats.time_launch += seconds_since(t);
+}
+
+void EngineHip::resolveProfile() {
+    for (PendingEvent &p : pending_events) {
+        float ms = 0;
+        bhhip_event_elapsed_ms(p.start, p.stop, &ms);
+        bhhip_event_destroy(p.start);
+        bhhip_event_destroy(p.stop);
+        p.rec.device_ms = ms;
+        stats.device_ms_by_strategy[p.rec.strategy] += ms;
+        last_launches.push_back(p.rec);
+    }
+    pending_events.clear();
 }
 
 void EngineHip::checkErrors() {
@@ -291,7 +299,6 @@ void EngineHip::checkErrors() {
 void EngineHip::handleExecution(BhIR *bhir) {
     const auto t_total = std::chrono::steady_clock::now();
     ++stats.flushes;
-    last_launches.clear();
     std::set<bh_base *> frees;
     std::vector<bh_instruction *> instr_list = jitk::remove_non_computed_system_instr(bhir->instr_list, frees);
     for (bh_base *b : frees) release(b);
@@ -512,6 +519,7 @@ public:
             engine.resetStatistics();
             engine.profile = true;
         } else if (msg == "statistic") {
+            engine.resolveProfile();
             ss << engine.statistics();
         } else if (msg == "info") {
             ss << engine.info();
@@ -529,9 +537,11 @@ public:
             for (const std::string &l : engine.dry_run_log) ss << l << "\n";
             engine.dry_run_log.clear();
         } else if (msg == "launches") {
+            engine.resolveProfile();
             for (const auto &r : engine.last_launches)
                 ss << r.strategy << " " << r.work_items << " " << r.grid << " " << r.device_ms << " " << r.kernel_hash
                    << "\n";
+            engine.last_launches.clear();
         } else if (msg.rfind("fuser_list:", 0) == 0) {
             std::vector<std::string> list;
             std::istringstream in(msg.substr(11));

@@ -77,6 +77,8 @@ public:
         std::string kernel_hash;
     };
     std::vector<LaunchRecord> last_launches;
+    /* Resolve the events of profiled launches into last_launches (synchronizes). */
+    void resolveProfile();
 
     jitk::FusionConfig fusion_config;
     bool profile = false;
@@ -97,6 +99,11 @@ private:
     uint64_t scan_scratch_bytes = 0;
     unsigned int *counters = nullptr;  // [0]: reduce ticket, [1]: error flag
     EngineStats stats;
+    struct PendingEvent {
+        void *start, *stop;
+        LaunchRecord rec;
+    };
+    std::vector<PendingEvent> pending_events;
 
     DeviceBuffer &buffer(bh_base *base);
     void ensureResident(bh_base *base);

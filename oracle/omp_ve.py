@@ -432,3 +432,172 @@ def replay(trace, keep=None):
     if keep is not None:
         return {b: data[b] for b in keep if b in data}
     return data
+
+
+# ---------------------------------------------------------------------------------------------
+# Fused OpenMP form (CPU baseline).  Runs of 1-D contiguous instructions of equal length are
+# emitted as one `#pragma omp parallel for` loop in which temporaries (created and freed inside
+# the run) are scalars and 1-D full reductions use OpenMP reduction clauses, like the kernels the
+# reference OpenMP engine generates for such blocks (ve/openmp/engine_openmp.cpp:274-329,
+# ve/openmp/openmp_util.hpp:27-94).  Everything else runs per instruction as in `replay`.
+
+_OMP_RED = {OP["ADD_REDUCE"]: "+", OP["MULTIPLY_REDUCE"]: "*", OP["MINIMUM_REDUCE"]: "min",
+            OP["MAXIMUM_REDUCE"]: "max", OP["BITWISE_AND_REDUCE"]: "&", OP["BITWISE_OR_REDUCE"]: "|",
+            OP["BITWISE_XOR_REDUCE"]: "^", OP["LOGICAL_AND_REDUCE"]: "&&", OP["LOGICAL_OR_REDUCE"]: "||"}
+_NO_FUSE = {OP["GATHER"], OP["SCATTER"], OP["COND_SCATTER"], OP["ADD_ACCUMULATE"], OP["MULTIPLY_ACCUMULATE"]}
+
+
+def _is_1d_unit(view, n):
+    return view is not None and len(view[2]) == 1 and view[2][0] == n and view[3][0] == 1
+
+
+def _fusible(rec, n):
+    opcode, ops, const = rec[1], rec[2], rec[3]
+    if opcode in _NO_FUSE:
+        return False
+    if opcode in REDUCE:
+        return opcode in _OMP_RED and len(ops[1][2]) == 1 and _is_1d_unit(ops[1], n)
+    return all(o is None or _is_1d_unit(o, n) for o in ops)
+
+
+def _principal_len(rec):
+    opcode, ops = rec[1], rec[2]
+    v = ops[1] if opcode in REDUCE else ops[0]
+    return v[2][0] if len(v[2]) == 1 else None
+
+
+class FusedProgram:
+    """Compile a trace in the fused OpenMP form; run its flush groups one at a time."""
+
+    def __init__(self, trace):
+        self.dtypes, self.nelem = {}, {}
+        for rec in trace:
+            if rec[0] == "new":
+                self.dtypes[rec[1]] = np.dtype(rec[2]).str
+                self.nelem[rec[1]] = rec[3]
+        self.groups = _split_groups(trace)
+        gen = _Gen(self.dtypes)
+        gen.nelem = self.nelem
+        src = [HEADER]
+        for gi, (recs, _, _) in enumerate(self.groups):
+            src.append("int g%d(void **d) {\n  int err = 0;" % gi)
+            src.extend(self._group_source(gen, recs))
+            src.append("  return err;\n}\n")
+        self.source = "\n".join(src)
+        self.lib = _compile(self.source)
+        self.data = {}
+        self._gi = 0
+
+    def _group_source(self, gen, recs):
+        ops_recs = [r for r in recs if r[0] == "op"]
+        frees = set(r[1] for r in recs if r[0] == "free")
+        # bases used per instruction position, to find run-local temporaries
+        uses = {}
+        for k, r in enumerate(ops_recs):
+            for o in r[2]:
+                if o is not None:
+                    uses.setdefault(o[0], set()).add(k)
+        out, k = [], 0
+        while k < len(ops_recs):
+            r = ops_recs[k]
+            n = _principal_len(r)
+            if n is None or not _fusible(r, n):
+                out.extend(gen.instr(r[1], r[2], r[3]))
+                k += 1
+                continue
+            run = [r]
+            red_outs = set([r[2][0][0]]) if r[1] in REDUCE else set()
+            j = k + 1
+            while j < len(ops_recs):
+                q = ops_recs[j]
+                if _principal_len(q) != n or not _fusible(q, n):
+                    break
+                if any(o is not None and o[0] in red_outs for o in q[2]):
+                    break
+                if q[1] in REDUCE:
+                    red_outs.add(q[2][0][0])
+                run.append(q)
+                j += 1
+            idx = set(range(k, j))
+            out.extend(self._run_source(gen, run, n, frees, uses, idx))
+            k = j
+        return out
+
+    def _run_source(self, gen, run, n, frees, uses, idx):
+        temps = set()
+        for r in run:
+            if r[1] in REDUCE:
+                continue
+            b = r[2][0][0]
+            if b in frees and uses[b] <= idx:
+                temps.add(b)
+        accs = {}
+        for r in run:
+            if r[1] in REDUCE:
+                accs[r[2][0][0]] = r
+        L = ["  {"]
+        for b, r in accs.items():
+            t = np.dtype(self.dtypes[b])
+            L.append("    %s acc%d = %s;" % (CTYPE[t.str], b, c_literal(t, sweep_identity(r[1], t))))
+        clause = "".join(" reduction(%s:acc%d)" % (_OMP_RED[r[1]], b) for b, r in accs.items())
+        L.append("    #pragma omp parallel for%s" % clause)
+        L.append("    for (int64_t i = 0; i < %d; ++i) {" % n)
+        for b in sorted(temps):
+            L.append("      %s t%d;" % (CTYPE[self.dtypes[b]], b))
+
+        def ref(o):
+            if o[0] in temps:
+                return "t%d" % o[0]
+            return "%s[%d + i]" % (gen.arr(o[0]), o[1])
+
+        for r in run:
+            opcode, ops, const = r[1], r[2], r[3]
+            if opcode in REDUCE:
+                t = np.dtype(self.dtypes[ops[0][0]])
+                L.append("      " + c_operation(opcode, [t, t], ["acc%d" % ops[0][0], ref(ops[1])]))
+            elif opcode == OP["RANDOM"]:
+                s, key = const[1]
+                L.append("      %s = oracle_random123(%dULL, %dULL, (uint64_t)(%d + i));" % (ref(ops[0]), s, key,
+                                                                                          ops[0][1]))
+            elif opcode == OP["RANGE"]:
+                L.append("      %s = %d + i;" % (ref(ops[0]), ops[0][1]))
+            else:
+                types, cops = [], []
+                for o in ops:
+                    if o is None:
+                        types.append(np.dtype(const[0]))
+                        cops.append(c_literal(const[0], const[1]))
+                    else:
+                        types.append(np.dtype(self.dtypes[o[0]]))
+                        cops.append(ref(o))
+                L.append("      " + c_operation(opcode, types, cops))
+        L.append("    }")
+        for b, r in accs.items():
+            L.append("    %s[%d] = acc%d;" % (gen.arr(b), r[2][0][1], b))
+        L.append("  }")
+        return L
+
+    def run_next_group(self):
+        """Execute the next flush group; returns its wall time in seconds (kernel execution only)."""
+        import time
+        recs, nrep, cond = self.groups[self._gi]
+        for rec in recs:
+            if rec[0] == "new":
+                self.data[rec[1]] = np.zeros(max(1, rec[3]), dtype=np.dtype(rec[2]))
+            elif rec[0] == "data":
+                self.data[rec[1]] = np.array(rec[2], dtype=np.dtype(self.dtypes[rec[1]])).reshape(-1)
+        maxbid = max(self.dtypes) + 1 if self.dtypes else 1
+        ptrs = (ctypes.c_void_p * maxbid)()
+        for b, arr in self.data.items():
+            ptrs[b] = arr.ctypes.data
+        fn = getattr(self.lib, "g%d" % self._gi)
+        fn.argtypes = [ctypes.c_void_p * maxbid]
+        t0 = time.perf_counter()
+        for _ in range(int(nrep)):
+            if fn(ptrs) != 0:
+                raise IndexError("oracle: gather/scatter index out of bounds")
+            if cond is not None and not bool(self.data[cond][0]):
+                break
+        dt = time.perf_counter() - t0
+        self._gi += 1
+        return dt
