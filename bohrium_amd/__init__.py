@@ -12,7 +12,7 @@ import numpy as np
 from . import _bh_api
 from . import opcodes
 from .bharray import (BhArray, array, empty, full, zeros, ones, arange, take, put, flush, ufunc,
-                      assign, asnumpy_many)
+                      assign, asnumpy_many, Iterator, get_iterator)
 from . import random
 
 _GLOBAL_UFUNCS = ["add", "subtract", "multiply", "true_divide", "divide", "floor_divide", "power", "absolute",

@@ -75,6 +75,7 @@ def _load():
     lib.bhc_message.restype = ctypes.c_char_p
     lib.bhc_extmethod.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(BhcView)]
     lib.bhc_set_flush_threshold.argtypes = [i64]
+    lib.bhc_op_slide.argtypes = [i32, i64, i64, i64, i64, i64, i64]
     lib.bhc_base_nelem.argtypes = [vp]
     lib.bhc_base_nelem.restype = i64
     _lib = lib
@@ -87,6 +88,10 @@ def _check(rc):
 
 
 _initialised = [False]
+try:
+    from . import _bhc as _fast  # CPython extension over libbhc (built by build.py)
+except ImportError:  # pragma: no cover - the ctypes path is complete on its own
+    _fast = None
 
 
 def init():
@@ -164,10 +169,18 @@ def new(dtype, nelem):
         _fake_handles.add(h)
         _rec("new", _bid(h), dtype.str, int(nelem))
         return h
-    h = lib().bhc_new(BH_TYPES[dtype], int(nelem))
-    if not h:
-        raise BohriumError(_lib.bhc_last_error().decode())
-    _rec("new", _bid(h), dtype.str, int(nelem))
+    lib()
+    if _fast is not None:
+        try:
+            h = _fast.new(BH_TYPES[dtype], int(nelem))
+        except RuntimeError as e:
+            raise BohriumError(str(e))
+    else:
+        h = _lib.bhc_new(BH_TYPES[dtype], int(nelem))
+        if not h:
+            raise BohriumError(_lib.bhc_last_error().decode())
+    if _trace is not None:
+        _rec("new", _bid(h), dtype.str, int(nelem))
     return h
 
 
@@ -184,37 +197,57 @@ def make_view(handle, start, shape, stride):
     return v
 
 
-def op(opcode, operands, const=None):
+def op(opcode, operands, const=None, slides=None):
     """Queue one instruction.  `operands` are (handle, start, shape, stride)
     tuples or None for the constant operand, whose value is `const`
-    (a numpy scalar of the constant's type)."""
-    n = len(operands)
-    arr = (BhcView * max(1, n))()
-    rec_ops = []
-    for i, o in enumerate(operands):
-        if o is None:
-            arr[i] = BhcView()
-            rec_ops.append(None)
-        else:
-            h, start, shape, stride = o
-            if not _record_only[0]:
-                arr[i] = make_view(h, start, shape, stride)
-            rec_ops.append((_bid(h), int(start), tuple(int(s) for s in shape), tuple(int(s) for s in stride)))
-    ctype, cptr, keep = -1, None, None
+    (a numpy scalar of the constant's type).  `slides` optionally gives, per
+    operand, a tuple of slide dims (rank, offset_change, shape_change, stride,
+    shape, step_delay) that move the view between repeats."""
+    ctype, cbytes = -1, None
     if const is not None:
         c = np.asarray(const)
         ctype = BH_TYPES[c.dtype]
-        keep = np.ascontiguousarray(c.reshape(1))
-        cptr = keep.ctypes.data_as(ctypes.c_void_p)
+        cbytes = c.tobytes()
     if not _record_only[0]:
-        _check(lib().bhc_op(int(opcode), n, arr, ctype, cptr))
-    _rec("op", int(opcode), rec_ops, None if const is None else (np.asarray(const).dtype.str, np.asarray(const).item()))
+        lib()
+        if _fast is not None:
+            try:
+                _fast.op(int(opcode), operands, ctype, cbytes)
+            except RuntimeError as e:
+                raise BohriumError(str(e))
+        else:
+            n = len(operands)
+            arr = (BhcView * max(1, n))()
+            for i, o in enumerate(operands):
+                arr[i] = BhcView() if o is None else make_view(*o)
+            keep = None if cbytes is None else ctypes.create_string_buffer(cbytes, 16)
+            _check(_lib.bhc_op(int(opcode), n, arr, ctype, keep))
+        if slides:
+            for k, dims in enumerate(slides):
+                for d in (dims or ()):
+                    if _fast is not None:
+                        _fast.op_slide(k, *[int(x) for x in d])
+                    else:
+                        _check(_lib.bhc_op_slide(k, *[ctypes.c_int64(int(x)) for x in d]))
+    if _trace is not None:
+        rec_ops = [None if o is None else (_bid(o[0]), int(o[1]), tuple(int(x) for x in o[2]),
+                                           tuple(int(x) for x in o[3])) for o in operands]
+        _rec("op", int(opcode), rec_ops, None if const is None else (np.asarray(const).dtype.str,
+                                                                     np.asarray(const).item()),
+             None if not slides else tuple(tuple(tuple(int(x) for x in d) for d in (dims or ())) for dims in slides))
 
 
 def random123(handle, start, shape, stride, r_start, r_key):
     if not _record_only[0]:
-        v = make_view(handle, start, shape, stride)
-        _check(lib().bhc_random123(ctypes.byref(v), ctypes.c_uint64(int(r_start)), ctypes.c_uint64(int(r_key))))
+        lib()
+        if _fast is not None:
+            try:
+                _fast.random123((handle, start, tuple(shape), tuple(stride)), int(r_start), int(r_key))
+            except RuntimeError as e:
+                raise BohriumError(str(e))
+        else:
+            v = make_view(handle, start, shape, stride)
+            _check(_lib.bhc_random123(ctypes.byref(v), ctypes.c_uint64(int(r_start)), ctypes.c_uint64(int(r_key))))
     _rec("op", 71, [(_bid(handle), int(start), tuple(shape), tuple(stride)), None], ("r123", (int(r_start), int(r_key))))
 
 
@@ -226,8 +259,15 @@ def free(handle):
         return
     if not _initialised[0]:
         return
-    _check(_lib.bhc_free(handle))
-    _rec("free", _bid(handle))
+    if _fast is not None:
+        try:
+            _fast.free(handle)
+        except RuntimeError as e:
+            raise BohriumError(str(e))
+    else:
+        _check(_lib.bhc_free(handle))
+    if _trace is not None:
+        _rec("free", _bid(handle))
 
 
 def sync(handle):

@@ -45,7 +45,7 @@ class BhArray:
 
     __array_priority__ = 100
 
-    def __init__(self, shape, dtype=np.float64, base=None, offset=0, strides=None):
+    def __init__(self, shape, dtype=np.float64, base=None, offset=0, strides=None, slides=()):
         if isinstance(shape, numbers.Integral):
             shape = (int(shape),)
         self.shape = tuple(int(s) for s in shape)
@@ -56,6 +56,9 @@ class BhArray:
         self.base = base if base is not None else BhBase(self.dtype, n)
         self.offset = int(offset)
         self.strides = tuple(strides) if strides is not None else _contiguous_strides(self.shape)
+        # Slide dims (rank, offset_change, shape_change, stride, shape, step_delay) applied between
+        # the repeats of do_while (views indexed by an Iterator).
+        self.slides = tuple(slides)
 
     # ------------------------------------------------------------ basics
     @property
@@ -119,8 +122,9 @@ class BhArray:
         return bool(self.item())
 
     # ------------------------------------------------------------ views
-    def _view(self, offset, shape, strides):
-        return BhArray(shape, base=self.base, offset=offset, strides=strides)
+    def _view(self, offset, shape, strides, slides=None):
+        return BhArray(shape, base=self.base, offset=offset, strides=strides,
+                       slides=self.slides if slides is None else slides)
 
     def __getitem__(self, key):
         if not isinstance(key, tuple):
@@ -130,6 +134,7 @@ class BhArray:
             fill = self.ndim - (len(key) - 1 - sum(1 for k in key if k is None))
             key = key[:i] + (slice(None),) * fill + key[i + 1:]
         offset, shape, strides = self.offset, [], []
+        slides = list(self.slides)
         dim = 0
         for k in key:
             if k is None:
@@ -145,6 +150,9 @@ class BhArray:
                 offset += start * st if cnt > 0 else 0
                 shape.append(cnt)
                 strides.append(st * step)
+            elif isinstance(k, Iterator):
+                offset += (k.val % n) * st
+                slides.append((0, k.step, 0, st, n, k.step_delay))
             elif isinstance(k, numbers.Integral):
                 k = int(k)
                 if k < 0:
@@ -158,7 +166,7 @@ class BhArray:
         for d in range(dim, self.ndim):
             shape.append(self.shape[d])
             strides.append(self.strides[d])
-        return self._view(offset, tuple(shape), tuple(strides))
+        return self._view(offset, tuple(shape), tuple(strides), tuple(slides))
 
     def __setitem__(self, key, value):
         assign(value, self[key])
@@ -278,6 +286,38 @@ class BhArray:
     def __itruediv__(self, o): return ufunc("true_divide")(self, o, out=self)
 
 
+class Iterator:
+    """An integer-like loop counter for do_while bodies: indexing an axis with it
+    gives a view that moves `step` elements along that axis at every repeat
+    (reference bridge/npbackend/bohrium/loop.py:287-312)."""
+
+    def __init__(self, val=0, step=1, step_delay=1):
+        self.val, self.step, self.step_delay = int(val), int(step), int(step_delay)
+
+    def __add__(self, k):
+        return Iterator(self.val + int(k), self.step, self.step_delay)
+
+    __radd__ = __add__
+
+    def __sub__(self, k):
+        return Iterator(self.val - int(k), self.step, self.step_delay)
+
+    def __rsub__(self, k):
+        return Iterator(int(k) - self.val, -self.step, self.step_delay)
+
+    def __mul__(self, k):
+        return Iterator(self.val * int(k), self.step * int(k), self.step_delay)
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return Iterator(-self.val, -self.step, self.step_delay)
+
+
+def get_iterator(val=0):
+    return Iterator(val, 1)
+
+
 # ---------------------------------------------------------------- helpers
 
 def _broadcast_view(a, shape):
@@ -298,11 +338,16 @@ def _broadcast_view(a, shape):
             strides.append(0)
         else:
             raise ValueError("cannot broadcast %s to %s" % (a.shape, shape))
-    return BhArray(tuple(shape), base=a.base, offset=a.offset, strides=tuple(strides))
+    return BhArray(tuple(shape), base=a.base, offset=a.offset, strides=tuple(strides), slides=a.slides)
 
 
 def _as_operand(x):
     return x.view_desc() if isinstance(x, BhArray) else None
+
+
+def _slides_of(arrays):
+    s = [a.slides if isinstance(a, BhArray) else () for a in arrays]
+    return s if any(s) else None
 
 
 def _same_memory(a, b):
@@ -312,9 +357,10 @@ def _same_memory(a, b):
 def ops_identity(src, dst):
     """dst[...] = src with a C cast (BH_IDENTITY)."""
     if isinstance(src, BhArray):
-        api.op(ops.IDENTITY, [dst.view_desc(), _broadcast_view(src, dst.shape).view_desc()])
+        api.op(ops.IDENTITY, [dst.view_desc(), _broadcast_view(src, dst.shape).view_desc()],
+               slides=_slides_of([dst, src]))
     else:
-        api.op(ops.IDENTITY, [dst.view_desc(), None], const=_const_scalar(src, dst.dtype))
+        api.op(ops.IDENTITY, [dst.view_desc(), None], const=_const_scalar(src, dst.dtype), slides=_slides_of([dst]))
 
 
 def _const_scalar(value, dtype):
@@ -365,11 +411,24 @@ def ones(shape, dtype=np.float64):
     return full(shape, 1, dtype)
 
 
-def arange(n, dtype=np.int64):
-    out = BhArray((int(n),), dtype)
+def arange(start, stop=None, step=1, dtype=None):
+    """Evenly spaced values (BH_RANGE, then an affine map when start/step are not 0/1)."""
+    if stop is None:
+        start, stop = 0, start
+    n = max(0, int(np.ceil((stop - start) / step)))
+    if dtype is None:
+        dtype = np.result_type(np.asarray(start), np.asarray(stop), np.asarray(step))
+        if dtype.kind in "iub":
+            dtype = np.dtype(np.int64)
+    dtype = np.dtype(dtype)
+    out = BhArray((n,), np.int64 if dtype.kind in "iu" else dtype)
     if n > 0:
         api.op(ops.RANGE, [out.view_desc()])
-    return out
+        if step != 1:
+            out = out * step
+        if start != 0:
+            out = out + start
+    return out if out.dtype == dtype else out.astype(dtype)
 
 
 def flush():
@@ -436,9 +495,20 @@ class Ufunc:
         self.np_ufunc = getattr(np, np_name or name)
         self.reduce_opcode = reduce_opcode
         self.accumulate_opcode = accumulate_opcode
+        self._cache = {}
         _UFUNCS[name] = self
 
     def _loop_dtypes(self, args):
+        key = tuple(a.dtype if isinstance(a, BhArray) else type(a) if isinstance(a, (bool, int, float, complex))
+                    else getattr(a, "dtype", None) for a in args)
+        hit = self._cache.get(key)
+        if hit is not None:
+            return hit
+        res = self._resolve(args)
+        self._cache[key] = res
+        return res
+
+    def _resolve(self, args):
         types = []
         for a in args:
             if isinstance(a, BhArray):
@@ -455,6 +525,10 @@ class Ufunc:
                 types.append(complex)
             else:
                 types.append(np.asarray(a).dtype)
+        if self.name in ("real", "imag"):
+            t = np.dtype(types[0]) if not isinstance(types[0], type) else np.dtype(np.float64)
+            out = np.dtype(np.float32) if t == np.complex64 else np.dtype(np.float64) if t.kind == "c" else t
+            return (t,), out
         res = self.np_ufunc.resolve_dtypes(tuple(types) + (None,))
         return res[:self.nin], res[self.nin]
 
@@ -497,7 +571,7 @@ class Ufunc:
         if callable(self.opcode):
             self.opcode(res, operands, const)
             return
-        api.op(self.opcode, descs, const=const)
+        api.op(self.opcode, descs, const=const, slides=_slides_of([res] + list(operands)))
 
     # -------------------------------------------------------- reduce
     def reduce(self, ary, axis=0, out=None):
@@ -530,7 +604,8 @@ class Ufunc:
         elif ary.shape[ax] == 1:
             ops_identity(ary[(slice(None),) * ax + (0,)], tmp)
         else:
-            api.op(self.reduce_opcode, [tmp.view_desc(), ary.view_desc(), None], const=np.int64(ax))
+            api.op(self.reduce_opcode, [tmp.view_desc(), ary.view_desc(), None], const=np.int64(ax),
+                   slides=_slides_of([tmp, ary]))
         if out is not None:
             assign(tmp, out)
             return out

@@ -69,8 +69,13 @@ def build(verbose=False):
          + [objs[s] for s in VE_HIP_CXX + VE_HIP_HIP]
          + ["-L" + LIB, "-lbh", rpath, "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64",
             "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    # CPython extension with the per-operation bridge calls.
+    import sysconfig
+    pyinc = sysconfig.get_paths()["include"]
+    _run(["g++"] + CXXFLAGS + ["-shared", "-I" + pyinc, os.path.join(CSRC, "bridge", "pybridge.cpp"),
+                               "-o", os.path.join(ROOT, "_bhc.so"), "-L" + LIB, "-lbhc", "-Wl,-rpath,$ORIGIN/lib"])
     if verbose:
-        print("built:", sorted(os.listdir(LIB)))
+        print("built:", sorted(os.listdir(LIB)) + ["_bhc.so"])
 
 
 if __name__ == "__main__":

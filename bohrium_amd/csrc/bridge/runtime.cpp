@@ -38,9 +38,10 @@ struct Runtime {
         stack = ComponentFace{top, 0};
     }
 
+    // Auto-flush happens before queueing, so the instruction queued last is always pending.
     void enqueue(bh_instruction &&instr) {
-        queue.push_back(std::move(instr));
         if (threshold > 0 && static_cast<int64_t>(queue.size()) >= threshold) flush(1, nullptr);
+        queue.push_back(std::move(instr));
     }
 
     void flush(uint64_t nrepeats, bh_base *cond) {
@@ -168,6 +169,26 @@ int bhc_random123(const bhc_view_t *out, uint64_t start, uint64_t key) {
         instr.constant.type = bh_type::R123;
         instr.constant.value.r123 = {start, key};
         rt().enqueue(std::move(instr));
+    });
+}
+
+int bhc_op_slide(int operand, int64_t rank, int64_t offset_change, int64_t shape_change, int64_t stride,
+                 int64_t shape, int64_t step_delay) {
+    return guarded([&] {
+        Runtime &r = rt();
+        if (r.queue.empty()) throw std::invalid_argument("bhc_op_slide: no queued instruction");
+        bh_instruction &i = r.queue.back();
+        if (operand < 0 || operand >= static_cast<int>(i.operand.size()) || i.operand[operand].isConstant())
+            throw std::invalid_argument("bhc_op_slide: bad operand");
+        if (step_delay < 1) throw std::invalid_argument("bhc_op_slide: step_delay must be >= 1");
+        bh_slide_dim d;
+        d.rank = rank;
+        d.offset_change = offset_change;
+        d.shape_change = shape_change;
+        d.stride = stride;
+        d.shape = shape;
+        d.step_delay = step_delay;
+        i.operand[operand].slides.dims.push_back(d);
     });
 }
 

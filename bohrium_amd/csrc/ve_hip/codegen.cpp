@@ -332,6 +332,8 @@ private:
     std::map<VKey, int> vid;
     std::map<OSKey, int> osid;
     std::vector<const bh_view *> os_views;
+    // Sliding views get an offset argument of their own, re-read from (origin, operand) at launch.
+    std::map<const bh_view *, std::pair<int64_t, int> > sliding_src;
     std::map<const LoopB *, int> lid;
     std::vector<const LoopB *> loops;
     std::vector<InstrPtr> const_instrs;
@@ -352,6 +354,15 @@ private:
         return id;
     }
     int os_id(const bh_view &v) {
+        if (v.hasSlide()) {
+            OSKey k{INT64_MIN, {static_cast<int64_t>(reinterpret_cast<uintptr_t>(&v))}};
+            auto it = osid.find(k);
+            if (it != osid.end()) return it->second;
+            const int id = static_cast<int>(os_views.size());
+            osid[k] = id;
+            os_views.push_back(&v);
+            return id;
+        }
         auto it = osid.find(oskey(v));
         if (it != osid.end()) return it->second;
         const int id = static_cast<int>(os_views.size());
@@ -461,9 +472,14 @@ void Writer::collect() {
             if (!it->isInstr()) stack.push_back(&it->getLoop());
     }
     // Register every view's offset-and-stride group in instruction order.
-    for (const InstrPtr &i : all)
-        for (const bh_view &v : i->operand)
-            if (!v.isConstant()) os_id(v);
+    for (const InstrPtr &i : all) {
+        for (size_t o = 0; o < i->operand.size(); ++o) {
+            const bh_view &v = i->operand[o];
+            if (v.isConstant()) continue;
+            os_id(v);
+            if (v.hasSlide()) sliding_src[&v] = {i->origin_id, static_cast<int>(o)};
+        }
+    }
 }
 
 void Writer::declare_scope(const LoopB &l, Scope &sc, int ind) {
@@ -604,6 +620,13 @@ std::string Writer::signature(const std::string &extra) {
     for (size_t k = 0; k < os_views.size(); ++k) {
         const bh_view &v = *os_views[k];
         lit("const int64_t vo" + std::to_string(k), &v.start, 8, 8);
+        auto src = sliding_src.find(&v);
+        if (src != sliding_src.end()) {
+            kp.args.back().view_origin = src->second.first;
+            kp.args.back().view_operand = src->second.second;
+        } else if (v.hasSlide()) {
+            kp.reusable = false;
+        }
         for (int64_t d = 0; d < v.ndim; ++d)
             lit("const int64_t vs" + std::to_string(k) + "_" + std::to_string(d), &v.stride[d], 8, 8);
     }
@@ -617,11 +640,16 @@ std::string Writer::signature(const std::string &extra) {
         const std::string cn = "c" + std::to_string(j);
         if (c.type == bh_type::R123) {
             lit("const uint64_t " + cn + "_start", &c.value.r123.start, 8, 8);
+            kp.args.back().const_origin = const_instrs[j]->origin_id;
+            kp.args.back().const_part = 0;
             lit("const uint64_t " + cn + "_key", &c.value.r123.key, 8, 8);
+            kp.args.back().const_origin = const_instrs[j]->origin_id;
+            kp.args.back().const_part = 1;
         } else {
             const int size = bh_type_size(c.type);
             const int align = bh_type_is_complex(c.type) ? size / 2 : size;
             lit("const " + std::string(hip_type(c.type)) + " " + cn, &c.value, size, align);
+            kp.args.back().const_origin = const_instrs[j]->origin_id;
         }
     }
     if (kp.uses_index_check) {

@@ -36,6 +36,17 @@ struct ArgSpec {
     std::vector<uint8_t> bytes;  // LITERAL value
     int size = 8;
     int align = 8;
+    /* A LITERAL that is an instruction's constant: the instruction's position in the
+     * flushed list (origin id) and which part of an R123 constant (0 start, 1 key, -1 whole).
+     * A cached plan re-reads these from the current flush. */
+    int64_t const_origin = -1;
+    int const_part = -1;
+    /* BASE_PTR: the base's ordinal in the flush (first-appearance order), set when cached. */
+    int base_ordinal = -1;
+    /* A LITERAL that is the start of a sliding view: read from operand `view_operand` of the
+     * instruction at position `view_origin` in the current flush. */
+    int64_t view_origin = -1;
+    int view_operand = -1;
 };
 
 struct KernelPlan {
@@ -52,6 +63,12 @@ struct KernelPlan {
     int64_t scratch_bytes = 0;
     bool uses_index_check = false;
     uint64_t work_items = 0;  // elements of the iteration space (for statistics / cost model features)
+    /* Filled when the plan is cached: ordinals of params/written bases and the compiled function. */
+    std::vector<int> param_ordinals, written_ordinals;
+    void *function = nullptr;
+    uint64_t source_hash = 0;
+    /* False when a sliding view's start cannot be re-read from the flush (plan not reusable). */
+    bool reusable = true;
 };
 
 /* Hooks the engine supplies to the writer. */

@@ -203,42 +203,56 @@ void EngineHip::ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need
     have = bytes;
 }
 
-void EngineHip::launch(const KernelPlan &plan) {
+void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2base,
+                           const std::vector<bh_instruction *> &instrs) {
     auto t = std::chrono::steady_clock::now();
-    void *fn = nullptr;
     if (dry_run) {
         check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), nullptr),
               "kernel compilation");
         stats.time_compile += seconds_since(t);
         char hex[32];
-        std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(fnv1a(plan.source)));
+        std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(plan.source_hash));
         dry_run_log.push_back(std::string(strategy_name(plan.strategy)) + " " + std::to_string(plan.grid) + " " +
                               std::to_string(plan.block) + " " + hex);
         ++stats.launches;
         ++stats.launches_by_strategy[strategy_name(plan.strategy)];
         return;
     }
-    check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &fn),
-          "kernel compilation");
-    stats.time_compile += seconds_since(t);
-
-    t = std::chrono::steady_clock::now();
-    for (bh_base *b : plan.params) ensureResident(b);
+    if (plan.function == nullptr) {
+        check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &plan.function),
+              "kernel compilation");
+        stats.time_compile += seconds_since(t);
+        t = std::chrono::steady_clock::now();
+    }
+    for (int o : plan.param_ordinals) ensureResident(ord2base[o]);
     if (plan.strategy == Strategy::FULL_REDUCE) ensureScratch(reduce_scratch, reduce_scratch_bytes, plan.scratch_bytes, false);
     if (plan.strategy == Strategy::SCAN) ensureScratch(scan_scratch, scan_scratch_bytes, plan.scratch_bytes, true);
 
-    std::vector<uint8_t> buf;
-    buf.reserve(512);
+    uint8_t buf[4096];
+    size_t n = 0;
     auto put = [&](const void *p, int size, int align) {
-        while (buf.size() % static_cast<size_t>(align)) buf.push_back(0);
-        const uint8_t *b = static_cast<const uint8_t *>(p);
-        buf.insert(buf.end(), b, b + size);
+        while (n % static_cast<size_t>(align)) buf[n++] = 0;
+        if (n + static_cast<size_t>(size) > sizeof(buf)) throw std::runtime_error("hip engine: kernel arguments exceed 4 KiB");
+        std::memcpy(buf + n, p, static_cast<size_t>(size));
+        n += static_cast<size_t>(size);
     };
     for (const ArgSpec &a : plan.args) {
         void *ptr = nullptr;
         switch (a.kind) {
-            case ArgSpec::LITERAL: put(a.bytes.data(), a.size, a.align); continue;
-            case ArgSpec::BASE_PTR: ptr = bufs.at(a.base).ptr; break;
+            case ArgSpec::LITERAL:
+                if (a.view_origin >= 0 && a.view_origin < static_cast<int64_t>(instrs.size())) {
+                    put(&instrs[a.view_origin]->operand[a.view_operand].start, 8, 8);
+                } else if (a.const_origin >= 0 && a.const_origin < static_cast<int64_t>(instrs.size())) {
+                    const bh_constant &c = instrs[a.const_origin]->constant;
+                    const void *src = a.const_part == 0 ? static_cast<const void *>(&c.value.r123.start)
+                                    : a.const_part == 1 ? static_cast<const void *>(&c.value.r123.key)
+                                                        : static_cast<const void *>(&c.value);
+                    put(src, a.size, a.align);
+                } else {
+                    put(a.bytes.data(), a.size, a.align);
+                }
+                continue;
+            case ArgSpec::BASE_PTR: ptr = bufs.at(ord2base[a.base_ordinal]).ptr; break;
             case ArgSpec::SCRATCH_PTR:
                 ptr = plan.strategy == Strategy::SCAN ? scan_scratch : reduce_scratch;
                 break;
@@ -247,7 +261,7 @@ void EngineHip::launch(const KernelPlan &plan) {
         }
         put(&ptr, 8, 8);
     }
-    while (buf.size() % 8) buf.push_back(0);
+    while (n % 8) buf[n++] = 0;
 
     void *ev0 = nullptr, *ev1 = nullptr;
     if (profile) {
@@ -255,21 +269,63 @@ void EngineHip::launch(const KernelPlan &plan) {
         bhhip_event_create(&ev1);
         bhhip_event_record(ev0);
     }
-    check(bhhip_launch(fn, plan.grid, plan.block, buf.data(), buf.size()), "kernel launch");
+    check(bhhip_launch(plan.function, plan.grid, plan.block, buf, n), "kernel launch");
     ++stats.launches;
     ++stats.launches_by_strategy[strategy_name(plan.strategy)];
-    for (bh_base *b : plan.written) {
-        DeviceBuffer &d = buffer(b);
+    for (int o : plan.written_ordinals) {
+        DeviceBuffer &d = buffer(ord2base[o]);
         d.dev_valid = true;
         d.host_valid = false;
     }
     if (profile) {
         bhhip_event_record(ev1);
         char hex[32];
-        std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(fnv1a(plan.source)));
+        std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(plan.source_hash));
         pending_events.push_back({ev0, ev1, {strategy_name(plan.strategy), plan.work_items, plan.grid, 0.0, hex}});
     }
     stats.time_launch += seconds_since(t);
+}
+
+std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
+                                std::unordered_map<const bh_base *, int> &base2ord) const {
+    std::string key;
+    key.reserve(instrs.size() * 96);
+    auto put = [&](int64_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    for (const bh_instruction *i : instrs) {
+        put(i->opcode);
+        put(i->constructor ? 1 : 0);
+        put(static_cast<int64_t>(i->operand.size()));
+        for (const bh_view &v : i->operand) {
+            if (v.isConstant()) {
+                put(-1);
+                put(static_cast<int64_t>(i->constant.type));
+                // A sweep's constant is its axis, which shapes the loop nest.
+                if (bh_opcode_is_sweep(i->opcode)) put(i->constant.get_int64());
+                continue;
+            }
+            auto it = base2ord.find(v.base);
+            int o;
+            if (it == base2ord.end()) {
+                o = static_cast<int>(ord2base.size());
+                base2ord[v.base] = o;
+                ord2base.push_back(v.base);
+                put(-2);
+                put(static_cast<int64_t>(v.base->dtype()));
+                put(v.base->nelem());
+            } else {
+                o = it->second;
+            }
+            put(o);
+            // A sliding view's start changes every repeat; it is a launch argument, not structure.
+            put(v.hasSlide() ? -3 : v.start);
+            put(v.ndim);
+            for (int64_t d = 0; d < v.ndim; ++d) {
+                put(v.shape[d]);
+                put(v.stride[d]);
+            }
+        }
+    }
+    return key;
 }
 
 void EngineHip::resolveProfile() {
@@ -305,27 +361,57 @@ void EngineHip::handleExecution(BhIR *bhir) {
     jitk::set_constructor_flag(instr_list);
 
     auto t = std::chrono::steady_clock::now();
-    std::vector<jitk::LoopB> kernels = jitk::get_kernel_list(instr_list, fusion_config);
-    stats.time_fuse += seconds_since(t);
-
-    bool index_checks = false;
+    std::vector<bh_base *> ord2base;
+    std::unordered_map<const bh_base *, int> base2ord;
+    const std::string key = flushKey(instr_list, ord2base, base2ord);
+    auto hit = flush_cache.find(key);
     static const bool debug = std::getenv("BH_HIP_DEBUG") != nullptr;
-    for (const jitk::LoopB &kernel : kernels) {
-        ++stats.kernels;
-        if (debug) std::fprintf(stderr, "[hip] kernel:\n%s", kernel.pprint().c_str());
-        if (!kernel.isSystemOnly()) {
-            t = std::chrono::steady_clock::now();
-            std::vector<KernelPlan> plans = plan_kernel(kernel, wcfg);
-            stats.time_codegen += seconds_since(t);
-            for (const KernelPlan &p : plans) {
-                if (debug) std::fprintf(stderr, "[hip] launch %s grid %llu\n%s\n", strategy_name(p.strategy),
-                                        static_cast<unsigned long long>(p.grid), std::getenv("BH_HIP_DEBUG")[0] == '2' ? p.source.c_str() : "");
-                launch(p);
-                index_checks |= p.uses_index_check;
+    if (hit == flush_cache.end()) {
+        ++flush_cache_misses;
+        std::vector<jitk::LoopB> kernels = jitk::get_kernel_list(instr_list, fusion_config);
+        stats.time_fuse += seconds_since(t);
+        t = std::chrono::steady_clock::now();
+        CachedFlush cf;
+        for (const jitk::LoopB &kernel : kernels) {
+            if (debug) std::fprintf(stderr, "[hip] kernel:\n%s", kernel.pprint().c_str());
+            CachedKernel ck;
+            if (!kernel.isSystemOnly()) {
+                ck.plans = plan_kernel(kernel, wcfg);
+                for (KernelPlan &p : ck.plans) {
+                    for (const ArgSpec &a : p.args)
+                        if (a.view_origin >= static_cast<int64_t>(instr_list.size())) p.reusable = false;
+                    cf.reusable &= p.reusable;
+                    for (bh_base *b : p.params) p.param_ordinals.push_back(base2ord.at(b));
+                    for (bh_base *b : p.written) p.written_ordinals.push_back(base2ord.at(b));
+                    for (ArgSpec &a : p.args)
+                        if (a.kind == ArgSpec::BASE_PTR) a.base_ordinal = base2ord.at(a.base);
+                    p.source_hash = fnv1a(p.source);
+                    cf.index_checks |= p.uses_index_check;
+                    if (debug)
+                        std::fprintf(stderr, "[hip] plan %s grid %llu\n%s\n", strategy_name(p.strategy),
+                                     static_cast<unsigned long long>(p.grid),
+                                     std::getenv("BH_HIP_DEBUG")[0] == '2' ? p.source.c_str() : "");
+                }
             }
+            for (bh_base *b : kernel.getAllFrees()) ck.free_ordinals.push_back(base2ord.at(b));
+            cf.kernels.push_back(std::move(ck));
         }
-        for (bh_base *b : kernel.getAllFrees()) release(b);
+        stats.time_codegen += seconds_since(t);
+        if (flush_cache.size() > 4096) flush_cache.clear();
+        if (!cf.reusable) flush_cache.erase(key);
+        hit = flush_cache.emplace(cf.reusable ? key : key + "#once", std::move(cf)).first;
+    } else {
+        ++flush_cache_hits;
+        stats.time_fuse += seconds_since(t);
     }
+    CachedFlush &cf = hit->second;
+    for (CachedKernel &ck : cf.kernels) {
+        ++stats.kernels;
+        for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
+        for (int o : ck.free_ordinals) release(ord2base[o]);
+    }
+    const bool index_checks = cf.index_checks;
+    if (!cf.reusable) flush_cache.erase(hit);
     if (index_checks) checkErrors();
     for (bh_base *b : bhir->getSyncs()) {
         copyToHost(b);
@@ -430,6 +516,7 @@ std::string EngineHip::statistics() const {
        << "\ntime_compile: " << stats.time_compile << "\ntime_launch: " << stats.time_launch
        << "\ntime_copy: " << stats.time_copy << "\ntime_total: " << stats.time_total << "\n";
     for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
+    ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
     ss << "pool_allocated: " << pool.allocated << "\npool_peak: " << pool.peak << "\npool_hits: " << pool.hits
        << "\npool_misses: " << pool.misses << "\n";
     return ss.str();

@@ -99,6 +99,22 @@ private:
     uint64_t scan_scratch_bytes = 0;
     unsigned int *counters = nullptr;  // [0]: reduce ticket, [1]: error flag
     EngineStats stats;
+    /* Plans of an already seen flush, keyed by its canonical encoding. */
+    struct CachedKernel {
+        std::vector<KernelPlan> plans;
+        std::vector<int> free_ordinals;
+    };
+    struct CachedFlush {
+        std::vector<CachedKernel> kernels;
+        bool index_checks = false;
+        bool reusable = true;
+    };
+    std::unordered_map<std::string, CachedFlush> flush_cache;
+    uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
+    std::string flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
+                         std::unordered_map<const bh_base *, int> &base2ord) const;
+    void launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2base,
+                    const std::vector<bh_instruction *> &instrs);
     struct PendingEvent {
         void *start, *stop;
         LaunchRecord rec;
@@ -109,7 +125,7 @@ private:
     void ensureResident(bh_base *base);
     void release(bh_base *base);
     void ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need, bool zero);
-    void launch(const KernelPlan &plan);
+
     void checkErrors();
 };
 

@@ -58,6 +58,27 @@ def black_scholes(n, steps=10, seed=42):
     return [black_scholes_step(S, K, T, i) for i in range(steps)]
 
 
+def black_scholes_repeat(S, K, T, steps=10, r=0.02, sigma=0.3, dt=0.01):
+    """The same `steps` time steps as one recorded loop body replayed by the engine
+    (do_while): step i reads the time offset dts[i] through a sliding view and writes
+    its sum to sums[i].  Returns the float64 array of the `steps` sums."""
+    dts = _b.arange(steps, dtype=np.float64) * dt
+    sums = _b.empty(steps, np.float64)
+
+    def body():
+        i = _b.get_iterator()
+        Tt = T + dts[i]
+        sq = ufunc("sqrt")(Tt)
+        d1 = (ufunc("log")(S / K) + (r + 0.5 * sigma * sigma) * Tt) / (sigma * sq)
+        d2 = d1 - sigma * sq
+        call = S * _cnd(d1) - K * ufunc("exp")(Tt * -r) * _cnd(d2)
+        sums[i] = call.sum()
+
+    from . import do_while
+    do_while(body, steps)
+    return sums
+
+
 def jacobi_grid(n, dtype=np.float32):
     """n x n grid: top row 40.0, the other borders -273.0, interior 0."""
     G = _b.zeros((n, n), dtype)

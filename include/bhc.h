@@ -53,6 +53,14 @@ bhc_base_t bhc_new(int dtype, int64_t nelem);
  * from `const_value` interpreted as `const_type` (a bh_type id). */
 int bhc_op(int64_t opcode, int nops, const bhc_view_t *views, int const_type, const void *const_value);
 int bhc_random123(const bhc_view_t *out, uint64_t start, uint64_t key);
+/* Make operand `operand` of the instruction queued last slide between the
+ * repeats of bhc_flush_and_repeat: its start moves by offset_change * stride
+ * elements per step (wrapping inside stride * shape) and dimension `rank` grows
+ * by shape_change, every `step_delay` repeats.  Same fields as the reference's
+ * bh_slide_dim (include/bohrium/bh_slide.hpp) set by _bh.slide_view
+ * (bridge/npbackend/bohrium/loop.py:522-548). */
+int bhc_op_slide(int operand, int64_t rank, int64_t offset_change, int64_t shape_change, int64_t stride,
+                 int64_t shape, int64_t step_delay);
 /* Queue BH_FREE; the base handle stays valid until the next flush has run. */
 int bhc_free(bhc_base_t base);
 int bhc_sync(bhc_base_t base);

@@ -235,9 +235,9 @@ def c_operation(opcode, types, ops):
 
 
 def _index(view, idx_names):
-    """start + sum(i_d * stride_d) with literal numbers."""
+    """start + sum(i_d * stride_d) with literal numbers (a sliding view's start is st[j])."""
     start, shape, stride = view[1], view[2], view[3]
-    terms = [str(start)]
+    terms = [str(start) if len(view) < 5 else "st[%d]" % view[4]]
     for name, st in zip(idx_names, stride):
         if st != 0:
             terms.append("%s*(%d)" % (name, st))
@@ -248,6 +248,20 @@ class _Gen:
     def __init__(self, dtypes):
         self.dtypes = dtypes  # bid -> dtype str
         self.lines = []
+        self.sliding = []     # [start, slide dims] of every sliding operand, indexed by j in st[j]
+
+    def with_slides(self, ops, slides):
+        """Tag sliding operands with their slot j (their start is read from st[j] at run time)."""
+        if not slides:
+            return ops
+        out = []
+        for o, dims in zip(ops, slides):
+            if o is not None and dims:
+                out.append(tuple(o) + (len(self.sliding),))
+                self.sliding.append([o[1], dims, 0])
+            else:
+                out.append(o)
+        return out
 
     def arr(self, bid):
         return "((%s*)d[%d])" % (CTYPE[self.dtypes[bid]], bid)
@@ -290,7 +304,7 @@ class _Gen:
             if opcode in REDUCE:
                 inner.append(c_operation(opcode, [t, t], ["acc", in_ref]))
                 out_names = outer if len(shape) > 1 else []
-                out_ref = self.ref(out, out_names) if out_names else "%s[%d]" % (self.arr(out[0]), out[1])
+                out_ref = self.ref(out, out_names) if out_names else "%s[%s]" % (self.arr(out[0]), _index(out, []))
                 body += self.loops([shape[axis]], inner, [names[axis]], indent="")
                 body.append("%s = acc;" % out_ref)
             else:
@@ -312,14 +326,14 @@ class _Gen:
         elif opcode == OP["GATHER"]:
             inp, idx = ops[1], ops[2]
             nb = self.nelem[inp[0]]
-            body = ["{ int64_t g = %d + (int64_t)%s;" % (inp[1], self.ref(idx, names)),
+            body = ["{ int64_t g = %s + (int64_t)%s;" % (_index(inp, []), self.ref(idx, names)),
                     "  if (g < 0 || g >= %d) err = 1; else %s = %s[g]; }" % (nb, self.ref(out, names),
                                                                           self.arr(inp[0]))]
         elif opcode in (OP["SCATTER"], OP["COND_SCATTER"]):
             val, idx = ops[1], ops[2]
             nb = self.nelem[out[0]]
             cond = "" if opcode == OP["SCATTER"] else "if (%s) " % self.ref(ops[3], names)
-            body = ["%s{ int64_t g = %d + (int64_t)%s;" % (cond, out[1], self.ref(idx, names)),
+            body = ["%s{ int64_t g = %s + (int64_t)%s;" % (cond, _index(out, []), self.ref(idx, names)),
                     "  if (g < 0 || g >= %d) err = 1; else %s[g] = %s; }" % (nb, self.arr(out[0]),
                                                                           self.ref(val, names))]
         else:
@@ -384,6 +398,26 @@ def _split_groups(trace):
     return groups
 
 
+def _slide(entry, st, j):
+    """Move one sliding operand to its next position (reference
+    include/bohrium/jitk/engines/dyn_view.hpp:21-67, without shape changes or resets)."""
+    dims, counter = entry[1], entry[2]
+    start = st[j]
+    for (rank, offset_change, shape_change, stride, shape, step_delay) in dims:
+        if step_delay == 1 or counter % step_delay == step_delay - 1:
+            if stride:
+                change = offset_change * stride
+                max_rel = stride * shape
+                rel = start % (stride * shape) + change
+                if rel < 0:
+                    change += max_rel
+                elif rel >= max_rel:
+                    change -= max_rel
+                start += change
+    st[j] = start
+    entry[2] = counter + 1
+
+
 def replay(trace, keep=None):
     """Execute `trace` on the CPU.  Returns {base id: numpy array} for every base
     still alive at the end (or only those in `keep`)."""
@@ -396,13 +430,20 @@ def replay(trace, keep=None):
     gen.nelem = nelem
     groups = _split_groups(trace)
     src = [HEADER]
+    slots = []
     for gi, (recs, _, _) in enumerate(groups):
-        src.append("int g%d(void **d) {\n  int err = 0;" % gi)
+        src.append("int g%d(void **d, int64_t *st) {\n  int err = 0;" % gi)
+        first = len(gen.sliding)
         for rec in recs:
             if rec[0] == "op":
-                src.extend(gen.instr(rec[1], rec[2], rec[3]))
+                ops = gen.with_slides(rec[2], rec[4] if len(rec) > 4 else None)
+                src.extend(gen.instr(rec[1], ops, rec[3]))
+        slots.append((first, len(gen.sliding)))
         src.append("  return err;\n}\n")
     lib = _compile("\n".join(src))
+    st = (ctypes.c_int64 * max(1, len(gen.sliding)))()
+    for j, (start, _, _) in enumerate(gen.sliding):
+        st[j] = start
     data = {}
     freed = set()
     maxbid = max(dtypes) + 1 if dtypes else 1
@@ -420,12 +461,14 @@ def replay(trace, keep=None):
         for b, arr in data.items():
             ptrs[b] = arr.ctypes.data
         fn = getattr(lib, "g%d" % gi)
-        fn.argtypes = [ctypes.c_void_p * maxbid]
+        fn.argtypes = [ctypes.c_void_p * maxbid, ctypes.c_int64 * max(1, len(gen.sliding))]
         for _ in range(int(nrep)):
-            if fn(ptrs) != 0:
+            if fn(ptrs, st) != 0:
                 raise IndexError("oracle: gather/scatter index out of bounds")
             if cond is not None and not bool(data[cond][0]):
                 break
+            for j in range(*slots[gi]):
+                _slide(gen.sliding[j], st, j)
         for b in list(freed):
             data.pop(b, None)
         freed.clear()

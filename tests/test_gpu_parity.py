@@ -263,3 +263,33 @@ def test_empty_and_ragged():
     got, want = run_both(prog)
     for g, w in zip(got, want):
         assert_parity(g, w, np.float64)
+
+
+def test_sliding_views_known_answers():
+    def prog():
+        a = bh.array(np.arange(1.0, 11.0))
+
+        def body(a):
+            i = bh.get_iterator(1)
+            a[i] += a[i - 1]
+        bh.do_while(body, 9, a)
+        b = bh.arange(1, 6)
+
+        def kernel(b):
+            i = bh.get_iterator(1)
+            b[i] *= b[i - 1]
+        bh.do_while(kernel, 4, b)
+        return [a, b]
+    got, want = run_both(prog)
+    np.testing.assert_array_equal(got[0], [1, 3, 6, 10, 15, 21, 28, 36, 45, 55])
+    np.testing.assert_array_equal(got[1], [1, 2, 6, 24, 120])
+    np.testing.assert_array_equal(want[1], [1, 2, 6, 24, 120])
+
+
+def test_black_scholes_repeat_full_size_sums():
+    """BASELINE config size (N = 2e7, 10 steps): engine vs oracle on the same bytecode."""
+    def prog():
+        S, K, T = workloads.black_scholes_inputs(20_000_000)
+        return workloads.black_scholes_repeat(S, K, T, steps=10)
+    got, want = run_both(prog)
+    assert_parity(got[0], want[0], np.float64, rtol=1e-12)

@@ -409,3 +409,41 @@ def test_jacobi_small_matches_numpy():
         np.testing.assert_allclose(res[1 + it], max(d, np.finfo(np.float32).tiny), rtol=1e-6)
         G[1:-1, 1:-1] = tmp
     np.testing.assert_allclose(res[0], G, rtol=1e-6, atol=1e-4)
+
+
+def test_sliding_views_triangular_numbers_known_answer():
+    """Reference docs acc_loop.rst:56-72: a[i+1] += a[i] with an iterator over 9 repeats."""
+    def prog():
+        a = bh.array(np.arange(1.0, 11.0))
+
+        def body(a):
+            i = bh.get_iterator(1)
+            a[i] += a[i - 1]
+        bh.do_while(body, 9, a)
+        return a
+    np.testing.assert_array_equal(run_oracle(prog)[0], [1, 3, 6, 10, 15, 21, 28, 36, 45, 55])
+
+
+def test_sliding_views_factorial_known_answer():
+    """Reference bohrium/loop.py:304-310 docstring: a[i] *= a[i-1] -> [1 2 6 24 120]."""
+    def prog():
+        a = bh.arange(1, 6)
+
+        def kernel(a):
+            i = bh.get_iterator(1)
+            a[i] *= a[i - 1]
+        bh.do_while(kernel, 4, a)
+        return a
+    np.testing.assert_array_equal(run_oracle(prog)[0], [1, 2, 6, 24, 120])
+
+
+def test_black_scholes_repeat_equals_loop_form():
+    from bohrium_amd import workloads
+    n = 513
+    loop = run_oracle(lambda: workloads.black_scholes(n, steps=5))
+
+    def prog():
+        S, K, T = workloads.black_scholes_inputs(n)
+        return workloads.black_scholes_repeat(S, K, T, steps=5)
+    rep = run_oracle(prog)[0]
+    np.testing.assert_allclose(rep, np.array(loop).ravel(), rtol=1e-15)
