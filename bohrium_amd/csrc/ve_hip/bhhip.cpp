@@ -26,11 +26,12 @@ std::string g_arch;
 std::string g_info;
 uint64_t g_compiles = 0, g_mem_hits = 0, g_disk_hits = 0;
 
+// One loaded code object per (source, options); its kernels are looked up by name.
 struct Loaded {
-    hipModule_t module;
-    hipFunction_t function;
+    hipModule_t module = nullptr;
+    std::unordered_map<std::string, hipFunction_t> functions;
 };
-std::unordered_map<uint64_t, Loaded> g_functions;
+std::unordered_map<uint64_t, Loaded> g_modules;
 
 int fail(hipError_t e, const char *what) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -174,13 +175,19 @@ int bhhip_compile(const char *prelude, const char *source, const char *name, con
     std::string full = std::string(prelude) + "\n" + source;
     std::string opts_key = g_arch;
     for (const char *o : kOptions) opts_key += std::string(" ") + o;
-    const uint64_t key = fnv1a(full, fnv1a(opts_key + "|" + name));
+    const uint64_t key = fnv1a(full, fnv1a(opts_key));
     if (function != nullptr) {
         std::lock_guard<std::mutex> lock(g_mu);
-        auto it = g_functions.find(key);
-        if (it != g_functions.end()) {
+        auto it = g_modules.find(key);
+        if (it != g_modules.end()) {
             ++g_mem_hits;
-            *function = reinterpret_cast<void *>(it->second.function);
+            auto f = it->second.functions.find(name);
+            if (f == it->second.functions.end()) {
+                hipFunction_t fn;
+                HIPCHK(hipModuleGetFunction(&fn, it->second.module, name), "hipModuleGetFunction");
+                f = it->second.functions.emplace(name, fn).first;
+            }
+            *function = reinterpret_cast<void *>(f->second);
             return 0;
         }
     }
@@ -224,13 +231,15 @@ int bhhip_compile(const char *prelude, const char *source, const char *name, con
     }
     if (function == nullptr) return 0;  // compile only
     Loaded l;
+    hipFunction_t fn;
     HIPCHK(hipModuleLoadData(&l.module, code.data()), "hipModuleLoadData");
-    HIPCHK(hipModuleGetFunction(&l.function, l.module, name), "hipModuleGetFunction");
+    HIPCHK(hipModuleGetFunction(&fn, l.module, name), "hipModuleGetFunction");
+    l.functions[name] = fn;
     {
         std::lock_guard<std::mutex> lock(g_mu);
-        g_functions[key] = l;
+        g_modules[key] = l;
     }
-    *function = reinterpret_cast<void *>(l.function);
+    *function = reinterpret_cast<void *>(fn);
     return 0;
 }
 

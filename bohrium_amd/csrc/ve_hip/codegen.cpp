@@ -1,6 +1,7 @@
 #include "codegen.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -304,9 +305,27 @@ struct Scope {
     const Scope *parent = nullptr;
     std::set<const bh_base *> tmps;
     std::set<VKey> srep;
+    // Scalar-replaced reduction outputs declared by this scope and stored when it closes.
+    std::vector<std::pair<const bh_view *, int64_t> > wb;
+    // Temporaries that stand for an accumulator of an enclosing skeleton (collapsed reductions).
+    std::map<const bh_base *, std::string> alias;
+    // Input views already loaded into a register of this name (unrolled grid-stride groups).
+    std::map<VKey, std::string> preload;
+    const std::string *preloadOf(const bh_view &v) const {
+        if (!preload.empty()) {
+            auto it = preload.find(vkey(v));
+            if (it != preload.end()) return &it->second;
+        }
+        return parent ? parent->preloadOf(v) : nullptr;
+    }
     bool isTmp(const bh_base *b) const { return tmps.count(b) || (parent && parent->isTmp(b)); }
     bool isSrep(const bh_view &v) const { return srep.count(vkey(v)) || (parent && parent->isSrep(v)); }
-    bool isArray(const bh_view &v) const { return !(isTmp(v.base) || isSrep(v)); }
+    const std::string *aliasOf(const bh_base *b) const {
+        auto it = alias.find(b);
+        if (it != alias.end()) return &it->second;
+        return parent ? parent->aliasOf(b) : nullptr;
+    }
+    bool isArray(const bh_view &v) const { return !(isTmp(v.base) || isSrep(v) || aliasOf(v.base)); }
     bool isDeclared(const bh_view &v) const { return isTmp(v.base) || isSrep(v); }
 };
 
@@ -339,11 +358,14 @@ private:
     std::vector<InstrPtr> const_instrs;
     std::map<int64_t, int> const_of_origin;
     std::set<const bh_base *> index_checked;  // bases needing their element count (gather/scatter)
+    // Instructions a skeleton implements itself (collapsed reductions and their identities).
+    std::set<const bh_instruction *> skip;
     // Instruction replaced by "store its input into `scan_sink`" in the scan skeleton.
     const bh_instruction *scan_instr = nullptr;
     std::string scan_sink;
 
     std::ostringstream body;
+    std::string main_body;  // FULL_REDUCE: the main kernel's body while `body` holds the finalize's
 
     void collect();
     int view_id(const bh_view &v) {
@@ -372,6 +394,7 @@ private:
     }
     std::string name(const Scope &sc, const bh_view &v) {
         std::ostringstream s;
+        if (const std::string *a = sc.aliasOf(v.base)) return *a;
         if (sc.isTmp(v.base)) {
             s << "t" << bid.at(v.base);
         } else if (sc.isSrep(v)) {
@@ -386,18 +409,21 @@ private:
         const int k = os_id(v);
         s << "vo" << k;
         if (!v.is_scalar()) {
+            // Strides are part of the plan key, so unit and zero strides are spelled out.
             for (int64_t d = 0; d < v.ndim; ++d) {
                 int64_t t = d >= hidden_axis ? d + 1 : d;
+                if (v.stride[d] == 0) continue;
                 if (t == off_axis)
                     s << " + (i" << t << " + (i" << t << " == 0 ? 0 : " << off << "))";
                 else
                     s << " + i" << t;
-                s << " * vs" << k << "_" << d;
+                if (v.stride[d] != 1) s << " * vs" << k << "_" << d;
             }
         }
         return s.str();
     }
     std::string ref(const Scope &sc, const bh_view &v, int hidden_axis = BH_MAXDIM) {
+        if (const std::string *p = sc.preloadOf(v)) return *p;
         std::string n = name(sc, v);
         if (sc.isArray(v)) n += "[" + index(v, hidden_axis) + "]";
         return n;
@@ -406,10 +432,21 @@ private:
 
     void declare_scope(const LoopB &l, Scope &sc, int ind);
     void write_back(const LoopB &l, Scope &sc, int ind);
+    void emit_block(const Block &b, Scope &sc, int ind);
     void emit_blocks(const LoopB &l, Scope &sc, int ind);
     void emit_body(const LoopB &l, const Scope *parent, int ind);
     void emit_instr(const bh_instruction &i, Scope &sc, int ind);
 
+    /* One element of a grid-stride loop: `decode(ind)` derives the loop indices from f_,
+     * `compute(ind, outer)` emits the body in a scope whose parent is `outer`. */
+    struct Element {
+        std::function<void(int)> decode;
+        std::function<void(int, const Scope *)> compute;
+        const LoopB *body_loop = nullptr;  // innermost loop whose read-only inputs may be preloaded
+        const Scope *outer = nullptr;
+    };
+    void grid_stride(const std::string &tot, int unroll, const Element &e);
+    std::vector<const bh_view *> preloadable(const LoopB &l);
     void emit_map();
     void emit_full_reduce();
     void emit_scan();
@@ -500,40 +537,40 @@ void Writer::declare_scope(const LoopB &l, Scope &sc, int ind) {
             const bh_view &v = i->operand[0];
             if (sc.isDeclared(v) || always_array.count(v.base)) continue;
             sc.srep.insert(vkey(v));
+            sc.wb.emplace_back(&v, i->sweep_axis());
             indent(body, ind);
             body << hip_type(v.base->dtype()) << " s" << bid.at(v.base) << "_" << view_id(v) << ";\n";
         }
     }
 }
 
-void Writer::write_back(const LoopB &l, Scope &sc, int ind) {
-    for (const InstrPtr &i : jitk::allLocalInstr(l)) {
-        if (i->operand.empty() || i->operand[0].isConstant()) continue;
-        const bh_view &v = i->operand[0];
-        if (!sc.isSrep(v)) continue;
+// Store the scope's scalar-replaced reduction outputs once, after the loops that sweep them.
+void Writer::write_back(const LoopB &, Scope &sc, int ind) {
+    for (const auto &w : sc.wb) {
+        const bh_view &v = *w.first;
         indent(body, ind);
-        body << "a" << bid.at(v.base) << "["
-             << index(v, bh_opcode_is_reduction(i->opcode) ? i->sweep_axis() : BH_MAXDIM) << "] = s"
-             << bid.at(v.base) << "_" << view_id(v) << ";\n";
-        sc.srep.erase(vkey(v));
+        body << "a" << bid.at(v.base) << "[" << index(v, static_cast<int>(w.second)) << "] = s" << bid.at(v.base) << "_"
+             << view_id(v) << ";\n";
     }
+    sc.wb.clear();
+}
+
+void Writer::emit_block(const Block &b, Scope &sc, int ind) {
+    if (b.isInstr()) {
+        if (b.getInstr() != nullptr && !bh_opcode_is_system(b.getInstr()->opcode) && !skip.count(b.getInstr().get()))
+            emit_instr(*b.getInstr(), sc, ind);
+        return;
+    }
+    const LoopB &c = b.getLoop();
+    indent(body, ind);
+    body << "for (int64_t i" << c.rank << " = 0; i" << c.rank << " < n" << lid.at(&c) << "; ++i" << c.rank << ") {\n";
+    emit_body(c, &sc, ind + 1);
+    indent(body, ind);
+    body << "}\n";
 }
 
 void Writer::emit_blocks(const LoopB &l, Scope &sc, int ind) {
-    for (const Block &b : l._block_list) {
-        if (b.isInstr()) {
-            if (b.getInstr() != nullptr && !bh_opcode_is_system(b.getInstr()->opcode))
-                emit_instr(*b.getInstr(), sc, ind);
-        } else {
-            const LoopB &c = b.getLoop();
-            indent(body, ind);
-            body << "for (int64_t i" << c.rank << " = 0; i" << c.rank << " < n" << lid.at(&c) << "; ++i" << c.rank
-                 << ") {\n";
-            emit_body(c, &sc, ind + 1);
-            indent(body, ind);
-            body << "}\n";
-        }
-    }
+    for (const Block &b : l._block_list) emit_block(b, sc, ind);
 }
 
 void Writer::emit_body(const LoopB &l, const Scope *parent, int ind) {
@@ -666,6 +703,69 @@ std::string Writer::signature(const std::string &extra) {
 }
 
 // ---- MAP: grid-stride over the flattened leading parallel ranks
+// Input views of an innermost loop that can be loaded before any of its instructions run:
+// reads of parameter arrays that no instruction of the loop writes (not gathered, scattered
+// or accumulated).  Distinct views in first-use order.
+std::vector<const bh_view *> Writer::preloadable(const LoopB &l) {
+    std::set<const bh_base *> written;
+    for (const InstrPtr &i : jitk::allInstr(l))
+        if (!i->operand.empty() && !i->operand[0].isConstant()) written.insert(i->operand[0].base);
+    std::vector<const bh_view *> out;
+    std::set<VKey> seen;
+    for (const InstrPtr &i : jitk::allInstr(l)) {
+        if (bh_opcode_is_system(i->opcode) || i->opcode == BH_GATHER) continue;
+        for (size_t o = 1; o < i->operand.size(); ++o) {
+            const bh_view &v = i->operand[o];
+            if (v.isConstant() || written.count(v.base) || always_array.count(v.base)) continue;
+            if (std::find(kp.params.begin(), kp.params.end(), v.base) == kp.params.end()) continue;
+            if (seen.insert(vkey(v)).second) out.push_back(&v);
+        }
+    }
+    return out;
+}
+
+// Grid-stride loop over the flat index f_ in [0, tot).  With unroll > 1, full groups of
+// `unroll` elements (strided by the grid) first load every read-only input of all of them
+// into registers and then compute them, so a wave has all of its loads in flight at once;
+// the remaining elements run one at a time.
+void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
+    body << "  const int64_t st_ = (int64_t)gridDim.x * BH_WG;\n";
+    body << "  int64_t g_ = (int64_t)blockIdx.x * BH_WG + threadIdx.x;\n";
+    if (unroll > 1) {
+        const std::vector<const bh_view *> reads =
+            e.body_loop != nullptr ? preloadable(*e.body_loop) : std::vector<const bh_view *>();
+        body << "  for (; g_ + " << unroll - 1 << " * st_ < " << tot << "; g_ += " << unroll << " * st_) {\n";
+        for (int u = 0; u < unroll; ++u)
+            for (size_t k = 0; k < reads.size(); ++k)
+                body << "    " << hip_type(reads[k]->base->dtype()) << " ld" << k << "_" << u << ";\n";
+        for (int u = 0; u < unroll; ++u) {
+            if (reads.empty()) break;
+            body << "    {\n      const int64_t f_ = g_ + " << u << " * st_;\n";
+            e.decode(3);
+            for (size_t k = 0; k < reads.size(); ++k)
+                body << "      ld" << k << "_" << u << " = " << ref(*e.outer, *reads[k]) << ";\n";
+            body << "    }\n";
+        }
+        for (int u = 0; u < unroll; ++u) {
+            Scope pre;
+            pre.parent = e.outer;
+            for (size_t k = 0; k < reads.size(); ++k)
+                pre.preload[vkey(*reads[k])] = "ld" + std::to_string(k) + "_" + std::to_string(u);
+            body << "    {\n      const int64_t f_ = g_ + " << u << " * st_;\n";
+            e.decode(3);
+            e.compute(3, &pre);
+            body << "    }\n";
+        }
+        body << "  }\n";
+    }
+    body << "  for (; g_ < " << tot << "; g_ += st_) {\n";
+    body << "    const int64_t f_ = g_;\n";
+    e.decode(2);
+    e.compute(2, e.outer);
+    body << "  }\n";
+}
+
+// ---- MAP: grid-stride over the flattened leading parallel ranks
 void Writer::emit_map() {
     const LoopB &l0 = unit._block_list.back().getLoop();
     const uint64_t p = std::max<uint64_t>(1, jitk::parallel_ranks(l0).first);
@@ -676,55 +776,132 @@ void Writer::emit_map() {
     const bool idx32 = total < (1ull << 31);
     Scope ksc;
     declare_scope(unit, ksc, 1);
-    body << "  int64_t tot_ = 1";
+    body << "  const int64_t tot_ = 1";
     for (uint64_t r = 0; r < p; ++r) body << " * n" << lid.at(chain[r]);
     body << ";\n";
-    body << "  for (int64_t f_ = (int64_t)blockIdx.x * BH_WG + threadIdx.x; f_ < tot_; f_ += (int64_t)gridDim.x * BH_WG) {\n";
-    if (p == 1) {
-        body << "    const int64_t i0 = f_;\n";
-    } else if (idx32) {
-        body << "    uint32_t r_ = (uint32_t)f_;\n";
-        for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r) {
-            body << "    const int64_t i" << r << " = (int64_t)(r_ % (uint32_t)n" << lid.at(chain[r]) << "); r_ /= (uint32_t)n"
-                 << lid.at(chain[r]) << ";\n";
+    // Only loops without inner sequential work are unrolled (their body is straight-line code).
+    const bool flat = p == 1 ? l0.isInnermost() : chain[p - 1]->isInnermost();
+    Element e;
+    e.outer = &ksc;
+    e.body_loop = flat ? chain[p - 1] : nullptr;
+    e.decode = [&](int ind) {
+        const std::string pad(2 * ind, ' ');
+        if (p == 1) {
+            body << pad << "const int64_t i0 = f_;\n";
+        } else if (idx32) {
+            body << pad << "uint32_t r_ = (uint32_t)f_;\n";
+            for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r)
+                body << pad << "const int64_t i" << r << " = (int64_t)(r_ % (uint32_t)n" << lid.at(chain[r])
+                     << "); r_ /= (uint32_t)n" << lid.at(chain[r]) << ";\n";
+            body << pad << "const int64_t i0 = (int64_t)r_;\n";
+        } else {
+            body << pad << "int64_t r_ = f_;\n";
+            for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r)
+                body << pad << "const int64_t i" << r << " = r_ % n" << lid.at(chain[r]) << "; r_ /= n"
+                     << lid.at(chain[r]) << ";\n";
+            body << pad << "const int64_t i0 = r_;\n";
         }
-        body << "    const int64_t i0 = (int64_t)r_;\n";
-    } else {
-        body << "    int64_t r_ = f_;\n";
-        for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r)
-            body << "    const int64_t i" << r << " = r_ % n" << lid.at(chain[r]) << "; r_ /= n" << lid.at(chain[r]) << ";\n";
-        body << "    const int64_t i0 = r_;\n";
-    }
-    // Open the scopes of the parallel loops; the innermost one carries the body.
-    std::vector<Scope> scopes(p);
-    const Scope *parent = &ksc;
-    for (uint64_t r = 0; r < p; ++r) {
-        scopes[r].parent = parent;
-        body << std::string(4 + 2 * r, ' ') << "{\n";
-        declare_scope(*chain[r], scopes[r], 3 + r);
-        if (r == p - 1) emit_blocks(*chain[r], scopes[r], 3 + r);
-        parent = &scopes[r];
-    }
-    for (int64_t r = static_cast<int64_t>(p) - 1; r >= 0; --r) {
-        write_back(*chain[r], scopes[r], 3 + r);
-        body << std::string(4 + 2 * r, ' ') << "}\n";
-    }
-    body << "  }\n";
+    };
+    e.compute = [&](int ind, const Scope *outer) {
+        // Open the scopes of the parallel loops; the innermost one carries the body.
+        std::vector<Scope> scopes(p);
+        const Scope *parent = outer;
+        for (uint64_t r = 0; r < p; ++r) {
+            scopes[r].parent = parent;
+            body << std::string(2 * ind + 2 * r, ' ') << "{\n";
+            declare_scope(*chain[r], scopes[r], ind + 1 + static_cast<int>(r));
+            if (r == p - 1) emit_blocks(*chain[r], scopes[r], ind + 1 + static_cast<int>(r));
+            parent = &scopes[r];
+        }
+        for (int64_t r = static_cast<int64_t>(p) - 1; r >= 0; --r) {
+            write_back(*chain[r], scopes[r], ind + 1 + static_cast<int>(r));
+            body << std::string(2 * ind + 2 * r, ' ') << "}\n";
+        }
+    };
+    grid_stride("tot_", flat ? cfg.map_unroll : 1, e);
     write_back(unit, ksc, 1);
     kp.grid = std::max<uint64_t>(1, std::min<uint64_t>((total + kp.block - 1) / kp.block, cfg.max_grid));
 }
 
-// ---- FULL_REDUCE: rank-0 reductions with a deterministic single-launch grid combine
+// A rank-0 reduction loop whose body is one innermost rank-1 loop that reduces rows into
+// temporaries, which rank 0 reduces again with the same operator (max_i(max_j x[i][j])),
+// is the full reduction of the 2-D iteration space.  Returns, per temporary, the rank-0
+// reduction it feeds, and the instructions the flat skeleton implements itself; empty
+// when the loop does not have that form.
+std::map<const bh_base *, InstrPtr> collapsible_pairs(const LoopB &l0, const std::set<bh_base *> &temps,
+                                                      std::set<const bh_instruction *> &skip) {
+    std::map<const bh_base *, InstrPtr> none, pairs, inner;
+    const LoopB *l1 = nullptr;
+    for (const Block &b : l0._block_list) {
+        if (b.isInstr()) continue;
+        if (l1 != nullptr) return none;
+        l1 = &b.getLoop();
+    }
+    if (l1 == nullptr || l1->rank != 1 || !l1->isInnermost()) return none;
+    for (const InstrPtr &s : l1->_sweeps) {
+        if (!bh_opcode_is_reduction(s->opcode) || s->sweep_axis() != 1 || !temps.count(s->operand[0].base)) return none;
+        inner[s->operand[0].base] = s;
+    }
+    if (inner.empty()) return none;
+    std::set<const bh_instruction *> sk;
+    for (const Block &b : l0._block_list) {
+        if (!b.isInstr()) continue;
+        const InstrPtr &i = b.getInstr();
+        if (bh_opcode_is_system(i->opcode)) continue;
+        if (i->opcode == BH_IDENTITY && i->operand[1].isConstant() && inner.count(i->operand[0].base)) {
+            sk.insert(i.get());
+            continue;
+        }
+        if (bh_opcode_is_reduction(i->opcode) && i->sweep_axis() == 0 && inner.count(i->operand[1].base) &&
+            inner.at(i->operand[1].base)->opcode == i->opcode && !pairs.count(i->operand[1].base)) {
+            pairs[i->operand[1].base] = i;
+            sk.insert(i.get());
+            continue;
+        }
+        return none;
+    }
+    if (pairs.size() != inner.size()) return none;
+    // The row temporaries may not be read anywhere else.
+    for (const InstrPtr &i : jitk::allInstr(l0)) {
+        if (sk.count(i.get()) || bh_opcode_is_system(i->opcode)) continue;
+        for (size_t o = 1; o < i->operand.size(); ++o)
+            if (!i->operand[o].isConstant() && inner.count(i->operand[o].base)) return none;
+    }
+    skip.insert(sk.begin(), sk.end());
+    return pairs;
+}
+
+// ---- FULL_REDUCE: rank-0 reductions as two launches on one stream.  The main kernel
+// leaves one partial per workgroup and reduction in scratch; the finalize kernel (one
+// workgroup) folds the partials in workgroup order, stores the results and then runs
+// the small scalar blocks that follow the sweeping loop in the unit (the tail) in
+// thread 0.  The kernel boundary orders the two, so there is no fence, atomic ticket
+// or counter to reset, the sum order is fixed, and the pair can be graph-captured.
 void Writer::emit_full_reduce() {
-    const LoopB &l0 = unit._block_list.back().getLoop();
-    kp.work_items = static_cast<uint64_t>(l0.size);
+    size_t pos = 0;
+    for (size_t k = 0; k < unit._block_list.size(); ++k)
+        if (!unit._block_list[k].isInstr() && !unit._block_list[k].getLoop()._sweeps.empty()) pos = k;
+    const LoopB &l0 = unit._block_list[pos].getLoop();
     std::vector<InstrPtr> sweeps;
     for (const InstrPtr &i : jitk::allLocalInstr(l0))
         if (bh_opcode_is_reduction(i->opcode) && i->sweep_axis() == 0) sweeps.push_back(i);
     std::sort(sweeps.begin(), sweeps.end(), [](const InstrPtr &a, const InstrPtr &b) { return a->origin_id < b->origin_id; });
 
-    const uint64_t grid =
-        std::max<uint64_t>(1, std::min<uint64_t>((static_cast<uint64_t>(l0.size) + kp.block - 1) / kp.block, cfg.max_grid));
+    std::set<bh_base *> all_temps = temps;
+    {
+        const std::set<bh_base *> lt = unit.getLocalTemps();
+        all_temps.insert(lt.begin(), lt.end());
+    }
+    const std::map<const bh_base *, InstrPtr> pairs = collapsible_pairs(l0, all_temps, skip);
+    const LoopB *l1 = nullptr;
+    uint64_t total = static_cast<uint64_t>(l0.size);
+    if (!pairs.empty()) {
+        for (const Block &b : l0._block_list)
+            if (!b.isInstr()) l1 = &b.getLoop();
+        total *= static_cast<uint64_t>(l1->size);
+    }
+    kp.work_items = total;
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((total + kp.block - 1) / kp.block, cfg.max_grid));
     kp.grid = grid;
 
     // Kernel scope: every rank-0 reduction output is a per-thread accumulator that starts at the identity.
@@ -746,65 +923,123 @@ void Writer::emit_full_reduce() {
         a.instr = s;
         a.t = t;
         a.is_tmp = ktmps.count(v.base) > 0 || temps.count(v.base) > 0;
+        const std::string ident = literal(jitk::sweep_identity(s->opcode, t));
         if (a.is_tmp) {
+            a.var = "t" + std::to_string(bid.at(v.base));
             if (!ksc.tmps.count(v.base)) {
                 ksc.tmps.insert(v.base);
-                body << "  " << hip_type(t) << " t" << bid.at(v.base) << " = " << literal(jitk::sweep_identity(s->opcode, t)) << ";\n";
+                body << "  " << hip_type(t) << " " << a.var << " = " << ident << ";\n";
             }
-            a.var = "t" + std::to_string(bid.at(v.base));
         } else {
-            ksc.srep.insert(vkey(v));
             a.var = "s" + std::to_string(bid.at(v.base)) + "_" + std::to_string(view_id(v));
-            body << "  " << hip_type(t) << " " << a.var << " = " << literal(jitk::sweep_identity(s->opcode, t)) << ";\n";
+            if (!ksc.srep.count(vkey(v))) {
+                ksc.srep.insert(vkey(v));
+                body << "  " << hip_type(t) << " " << a.var << " = " << ident << ";\n";
+            }
         }
         a.part_off = off;
         off += ((static_cast<int64_t>(grid) * bh_type_size(t) + 255) / 256) * 256;
         accs.push_back(a);
     }
     kp.scratch_bytes = off;
-    // Any other kernel-level temporaries.
     declare_scope(unit, ksc, 1);
-    body << "  for (int64_t i0 = (int64_t)blockIdx.x * BH_WG + threadIdx.x; i0 < n" << lid.at(&l0)
-         << "; i0 += (int64_t)gridDim.x * BH_WG) {\n";
-    {
-        Scope sc;
-        sc.parent = &ksc;
-        declare_scope(l0, sc, 2);
-        emit_blocks(l0, sc, 2);
-        write_back(l0, sc, 2);
+    if (!pairs.empty()) {
+        // One flat grid-stride loop over (i0, i1); each row temporary is the accumulator
+        // of the rank-0 reduction it feeds.
+        for (const auto &p : pairs)
+            for (const Acc &a : accs)
+                if (a.instr == p.second) ksc.alias[p.first] = a.var;
+        const std::string n0 = "n" + std::to_string(lid.at(&l0)), n1 = "n" + std::to_string(lid.at(l1));
+        body << "  const int64_t tot_ = " << n0 << " * " << n1 << ";\n";
+        Element e;
+        e.outer = &ksc;
+        e.body_loop = l1;
+        e.decode = [&](int ind) {
+            const std::string pad(2 * ind, ' ');
+            if (total < (1ull << 31)) {
+                body << pad << "const int64_t i0 = (int64_t)((uint32_t)f_ / (uint32_t)" << n1 << ");\n";
+                body << pad << "const int64_t i1 = (int64_t)((uint32_t)f_ - (uint32_t)i0 * (uint32_t)" << n1 << ");\n";
+            } else {
+                body << pad << "const int64_t i0 = f_ / " << n1 << ";\n" << pad << "const int64_t i1 = f_ - i0 * " << n1
+                     << ";\n";
+            }
+        };
+        e.compute = [&](int ind, const Scope *outer) {
+            Scope sc0;
+            sc0.parent = outer;
+            Scope sc;
+            sc.parent = &sc0;
+            declare_scope(*l1, sc, ind);
+            emit_blocks(*l1, sc, ind);
+            write_back(*l1, sc, ind);
+        };
+        grid_stride("tot_", cfg.reduce_unroll, e);
+    } else {
+        Element e;
+        e.outer = &ksc;
+        e.body_loop = l0.isInnermost() ? &l0 : nullptr;
+        e.decode = [&](int ind) { body << std::string(2 * ind, ' ') << "const int64_t i0 = f_;\n"; };
+        e.compute = [&](int ind, const Scope *outer) {
+            Scope sc;
+            sc.parent = outer;
+            declare_scope(l0, sc, ind);
+            emit_blocks(l0, sc, ind);
+            write_back(l0, sc, ind);
+        };
+        grid_stride("n" + std::to_string(lid.at(&l0)), l0.isInnermost() ? cfg.reduce_unroll : 1, e);
     }
-    body << "  }\n";
     body << "  __shared__ __attribute__((aligned(16))) unsigned char bh_lds_[BH_WAVES * 16];\n";
-    body << "  __shared__ bool bh_last_;\n";
-    for (size_t k = 0; k < accs.size(); ++k) {
-        const Acc &a = accs[k];
+    auto op_lambda = [&](const Acc &a) {
         const std::string T = hip_type(a.t);
-        body << "  { auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
-             << combine_expr(a.instr->opcode, "x", "y") << "; };\n";
+        return "auto op_ = [](" + T + " x, " + T + " y) -> " + T + " { return " +
+               combine_expr(a.instr->opcode, "x", "y") + "; };";
+    };
+    std::set<std::string> done;
+    for (const Acc &a : accs) {
+        if (!done.insert(a.var).second) continue;
+        const std::string T = hip_type(a.t);
+        body << "  { " << op_lambda(a) << "\n";
         body << "    " << a.var << " = bh_block_reduce(" << a.var << ", op_, (" << T << " *)bh_lds_);\n";
         body << "    if (threadIdx.x == 0) ((" << T << " *)(bh_scratch + " << a.part_off << "))[blockIdx.x] = " << a.var
-             << "; }\n";
+             << ";\n";
+        body << "    __syncthreads(); }\n";
     }
-    body << "  __threadfence();\n  __syncthreads();\n";
-    body << "  if (threadIdx.x == 0) bh_last_ = (atomicAdd(bh_counter, 1u) == gridDim.x - 1);\n";
-    body << "  __syncthreads();\n";
-    body << "  if (!bh_last_) return;\n  __threadfence();\n";
-    for (size_t k = 0; k < accs.size(); ++k) {
-        const Acc &a = accs[k];
+    main_body = body.str();
+    body.str("");
+    body.clear();
+
+    // Finalize kernel: fold the partials, store, run the tail in thread 0.
+    Scope fsc;
+    done.clear();
+    body << "  __shared__ __attribute__((aligned(16))) unsigned char bh_lds_[BH_WAVES * 16];\n";
+    for (const Acc &a : accs) {
+        if (!done.insert(a.var).second) continue;
         const std::string T = hip_type(a.t);
-        body << "  { auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
-             << combine_expr(a.instr->opcode, "x", "y") << "; };\n";
+        const bh_view &v = a.instr->operand[0];
+        if (a.is_tmp) {
+            fsc.tmps.insert(v.base);
+        } else {
+            fsc.srep.insert(vkey(v));
+        }
+        body << "  " << T << " " << a.var << ";\n";
+        body << "  { " << op_lambda(a) << "\n";
         body << "    const " << T << " *p_ = (const " << T << " *)(bh_scratch + " << a.part_off << ");\n";
         body << "    " << T << " v_ = " << literal(jitk::sweep_identity(a.instr->opcode, a.t)) << ";\n";
-        body << "    for (unsigned j_ = threadIdx.x; j_ < gridDim.x; j_ += BH_WG) v_ = op_(v_, p_[j_]);\n";
-        body << "    v_ = bh_block_reduce(v_, op_, (" << T << " *)bh_lds_);\n";
-        if (!a.is_tmp) {
-            const bh_view &v = a.instr->operand[0];
-            body << "    if (threadIdx.x == 0) a" << bid.at(v.base) << "[" << index(v) << "] = v_;\n";
-        }
-        body << "  }\n";
+        body << "    for (unsigned j_ = threadIdx.x; j_ < " << grid << "u; j_ += BH_WG) v_ = op_(v_, p_[j_]);\n";
+        body << "    " << a.var << " = bh_block_reduce(v_, op_, (" << T << " *)bh_lds_);\n";
+        body << "    __syncthreads(); }\n";
     }
-    body << "  if (threadIdx.x == 0) *bh_counter = 0u;\n";
+    body << "  if (threadIdx.x != 0) return;\n";
+    done.clear();
+    for (const Acc &a : accs) {
+        if (a.is_tmp || !done.insert(a.var).second) continue;
+        const bh_view &v = a.instr->operand[0];
+        body << "  a" << bid.at(v.base) << "[" << index(v) << "] = " << a.var << ";\n";
+    }
+    declare_scope(unit, fsc, 1);
+    for (size_t k = pos + 1; k < unit._block_list.size(); ++k) emit_block(unit._block_list[k], fsc, 1);
+    write_back(unit, fsc, 1);
+    kp.fin_name = "bh_kernel_fin";
 }
 
 // ---- SCAN: decoupled look-back over tiles of BH_WG * items elements
@@ -917,20 +1152,22 @@ KernelPlan Writer::plan(Strategy s) {
         case Strategy::SERIAL: emit_serial(); break;
     }
     std::string extra;
-    if (s == Strategy::FULL_REDUCE) {
-        extra = "unsigned char *__restrict__ bh_scratch,\n    unsigned int *__restrict__ bh_counter";
-    } else if (s == Strategy::SCAN) {
-        extra = "unsigned char *__restrict__ bh_scratch";
-    }
+    if (s == Strategy::FULL_REDUCE || s == Strategy::SCAN) extra = "unsigned char *__restrict__ bh_scratch";
     std::ostringstream src;
-    src << signature(extra) << " {\n" << body.str() << "}\n";
+    const std::string sig = signature(extra);
+    if (s == Strategy::FULL_REDUCE) {
+        // Both kernels take the same arguments, so one packed buffer serves both launches.
+        std::string fsig = sig;
+        const std::string from = " " + kp.name + "(";
+        fsig.replace(fsig.find(from), from.size(), " " + kp.fin_name + "(");
+        src << sig << " {\n" << main_body << "}\n\n" << fsig << " {\n" << body.str() << "}\n";
+    } else {
+        src << sig << " {\n" << body.str() << "}\n";
+    }
     if (s == Strategy::FULL_REDUCE) {
         ArgSpec a;
         a.kind = ArgSpec::SCRATCH_PTR;
         kp.args.push_back(a);
-        ArgSpec c;
-        c.kind = ArgSpec::COUNTER_PTR;
-        kp.args.push_back(c);
     } else if (s == Strategy::SCAN) {
         ArgSpec a;
         a.kind = ArgSpec::SCRATCH_PTR;
@@ -1003,6 +1240,89 @@ bool identities_of(const std::vector<Block> &ids, const LoopB &l0) {
 }
 } // namespace
 
+namespace {
+// Iterations one thread runs for a block: the product of its nested loop sizes.
+uint64_t serial_cost(const Block &b) {
+    if (b.isInstr()) return 1;
+    const LoopB &l = b.getLoop();
+    uint64_t inner = 1;
+    for (const Block &c : l._block_list) inner = std::max(inner, serial_cost(c));
+    return static_cast<uint64_t>(std::max<int64_t>(l.size, 0)) * inner;
+}
+
+// Blocks this small run in thread 0 of a FULL_REDUCE finalize kernel instead of a launch of their own.
+constexpr uint64_t kTailLimit = 256;
+bool tail_block(const Block &b) { return serial_cost(b) <= kTailLimit; }
+
+bool is_full_reduce_loop(const Block &b) {
+    return !b.isInstr() && !b.getLoop()._sweeps.empty() && classify_root_block(b.getLoop()) == Strategy::FULL_REDUCE;
+}
+
+// True when the kernel's last sizeable block is a FULL_REDUCE loop (only tail blocks after it).
+bool open_tail(const LoopB &kernel) {
+    for (auto it = kernel._block_list.rbegin(); it != kernel._block_list.rend(); ++it) {
+        if (is_full_reduce_loop(*it)) return true;
+        if (!(it->isSystemOnly() || tail_block(*it))) return false;
+    }
+    return false;
+}
+} // namespace
+
+namespace {
+// Bases a kernel reads and writes (a BH_FREE counts as a write).
+struct Access {
+    std::set<const bh_base *> reads, writes;
+    explicit Access(const LoopB &k) {
+        for (const InstrPtr &i : jitk::allInstr(k)) {
+            for (size_t o = 0; o < i->operand.size(); ++o) {
+                const bh_view &v = i->operand[o];
+                if (v.isConstant()) continue;
+                if (o == 0 || i->opcode == BH_FREE) {
+                    writes.insert(v.base);
+                } else {
+                    reads.insert(v.base);
+                }
+            }
+        }
+    }
+    // True when running `this` before `m` instead of after it could change a result.
+    bool conflicts(const Access &m) const {
+        for (const bh_base *b : m.writes)
+            if (reads.count(b) || writes.count(b)) return true;
+        for (const bh_base *b : writes)
+            if (m.reads.count(b)) return true;
+        return false;
+    }
+};
+} // namespace
+
+std::vector<LoopB> fold_serial_tails(std::vector<LoopB> kernels) {
+    std::vector<LoopB> out;
+    for (LoopB &k : kernels) {
+        bool small = !k._block_list.empty();
+        for (const Block &b : k._block_list) small &= b.isSystemOnly() || tail_block(b);
+        size_t target = out.size();
+        if (small) {
+            // The nearest earlier kernel with an open tail, if `k` can move before the kernels after it.
+            const Access ak(k);
+            for (size_t m = out.size(); m-- > 0;) {
+                if (open_tail(out[m])) {
+                    target = m;
+                    break;
+                }
+                if (ak.conflicts(Access(out[m]))) break;
+            }
+        }
+        if (target < out.size()) {
+            for (Block &b : k._block_list) out[target]._block_list.push_back(std::move(b));
+            out[target].metadataUpdate();
+        } else {
+            out.push_back(std::move(k));
+        }
+    }
+    return out;
+}
+
 std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg) {
     std::vector<KernelPlan> ret;
     auto emit = [&](LoopB unit, Strategy s) {
@@ -1014,7 +1334,9 @@ std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg
     // loop are folded into it when its skeleton seeds the sweep with the identity
     // itself (FULL_REDUCE, SCAN); otherwise they run first.
     std::vector<Block> pending;
-    for (const Block &b : kernel._block_list) {
+    const std::vector<Block> &blocks = kernel._block_list;
+    for (size_t bi = 0; bi < blocks.size(); ++bi) {
+        const Block &b = blocks[bi];
         if (b.isInstr()) {
             pending.push_back(b);
             continue;
@@ -1042,6 +1364,10 @@ std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg
         LoopB unit(-1, 1, scalar_ids);
         unit._block_list.push_back(b);
         if (s == Strategy::MAP && unit._block_list.size() > 1) s = Strategy::SERIAL;
+        if (s == Strategy::FULL_REDUCE) {
+            while (bi + 1 < blocks.size() && (blocks[bi + 1].isSystemOnly() || tail_block(blocks[bi + 1])))
+                unit._block_list.push_back(blocks[++bi]);
+        }
         emit(unit, s);
     }
     for (const Block &p : pending) {

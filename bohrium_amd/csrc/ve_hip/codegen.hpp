@@ -5,8 +5,9 @@
  *   MAP          grid-stride loop over the flattened leading parallel ranks;
  *                inner ranks (and inner sweeps) run as sequential loops.
  *   FULL_REDUCE  rank-0 reductions: per-thread accumulators, wave64 butterfly,
- *                LDS across waves, per-workgroup partials and a last-workgroup
- *                combine in workgroup order (deterministic, one launch).
+ *                LDS across waves, per-workgroup partials; a one-workgroup
+ *                finalize kernel folds them in workgroup order (deterministic)
+ *                and runs the small scalar blocks that follow in thread 0.
  *   SCAN         rank-0 add/multiply accumulate: wave64 shuffle scan, LDS
  *                across waves and a decoupled look-back across workgroups.
  *   SERIAL       anything else, executed by one thread (always correct).
@@ -53,6 +54,9 @@ struct KernelPlan {
     Strategy strategy = Strategy::SERIAL;
     std::string source;  // kernel source without the prelude
     std::string name;
+    /* FULL_REDUCE: a second kernel of the same module, launched with one workgroup right
+     * after `name` with the same arguments (folds the per-workgroup partials). */
+    std::string fin_name;
     std::vector<ArgSpec> args;
     uint64_t grid = 1;
     uint32_t block = 256;
@@ -66,6 +70,7 @@ struct KernelPlan {
     /* Filled when the plan is cached: ordinals of params/written bases and the compiled function. */
     std::vector<int> param_ordinals, written_ordinals;
     void *function = nullptr;
+    void *fin_function = nullptr;
     uint64_t source_hash = 0;
     /* False when a sliding view's start cannot be re-read from the flush (plan not reusable). */
     bool reusable = true;
@@ -75,8 +80,14 @@ struct KernelPlan {
 struct WriterConfig {
     int max_grid = 2048;     // workgroups for grid-stride skeletons
     int scan_items = 8;      // elements per thread per scan tile
+    int map_unroll = 1;      // grid-stride elements per thread issued together (MAP)
+    int reduce_unroll = 1;   // the same for FULL_REDUCE
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
+
+/* Append each kernel made only of small scalar blocks to the kernel before it when
+ * that one ends in a FULL_REDUCE loop: its finalize kernel then runs them. */
+std::vector<jitk::LoopB> fold_serial_tails(std::vector<jitk::LoopB> kernels);
 
 /* Split a jitk kernel (rank -1 LoopB) into launch plans. */
 std::vector<KernelPlan> plan_kernel(const jitk::LoopB &kernel, const WriterConfig &cfg);

@@ -82,6 +82,7 @@ void DevicePool::trim(uint64_t keep) {
         auto it = std::prev(_free.end());  // largest first
         bhhip_synchronize();
         bhhip_free(it->second);
+        ++frees;
         _cached -= it->first;
         allocated -= it->first;
         _free.erase(it);
@@ -111,7 +112,11 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     fusion_config.merge_filter = [](const jitk::LoopB &a, const jitk::LoopB &b) { return root_merge_allowed(a, b); };
     wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
     wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
+    wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
+    wcfg.reduce_unroll = std::max(1, cfg.defaultGet<int>("reduce_unroll", 1));
+    unflatten = cfg.defaultGet<bool>("unflatten_reductions", true);
     resident = cfg.defaultGet<bool>("resident", true);
+    graph_repeats = cfg.defaultGet<bool>("graph_repeats", true);
     profile = cfg.defaultGet<bool>("prof", false);
     cache_dir = cfg.defaultGet<std::string>("cache_dir", "");
     if (cache_dir.empty() || cache_dir == "NONE") cache_dir = default_cache_dir();
@@ -127,6 +132,8 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
 EngineHip::~EngineHip() {
     if (dry_run) return;
     bhhip_synchronize();
+    for (auto &kv : graph_cache) bhhip_graph_destroy(kv.second);
+    graph_cache.clear();
     for (auto &kv : bufs)
         if (!kv.second.external) pool.release(kv.second.ptr, kv.second.bytes);
     bufs.clear();
@@ -198,6 +205,7 @@ void EngineHip::ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need
     }
     void *p = nullptr;
     check(bhhip_malloc(&p, bytes), "scratch allocation");
+    ++scratch_allocs;
     if (zero) check(bhhip_memzero(p, bytes), "scratch zeroing");
     ptr = static_cast<unsigned char *>(p);
     have = bytes;
@@ -210,6 +218,7 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
         check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), nullptr),
               "kernel compilation");
         stats.time_compile += seconds_since(t);
+        if (!plan.fin_name.empty()) ++stats.launches;
         char hex[32];
         std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(plan.source_hash));
         dry_run_log.push_back(std::string(strategy_name(plan.strategy)) + " " + std::to_string(plan.grid) + " " +
@@ -221,6 +230,10 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
     if (plan.function == nullptr) {
         check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &plan.function),
               "kernel compilation");
+        if (!plan.fin_name.empty())
+            check(bhhip_compile(kPrelude, plan.source.c_str(), plan.fin_name.c_str(), cache_dir.c_str(),
+                                &plan.fin_function),
+                  "kernel compilation");
         stats.time_compile += seconds_since(t);
         t = std::chrono::steady_clock::now();
     }
@@ -271,6 +284,12 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
     }
     check(bhhip_launch(plan.function, plan.grid, plan.block, buf, n), "kernel launch");
     ++stats.launches;
+    if (recording != nullptr) recording->push_back({plan.function, plan.grid, plan.block, {buf, buf + n}});
+    if (plan.fin_function != nullptr) {
+        check(bhhip_launch(plan.fin_function, 1, plan.block, buf, n), "kernel launch");
+        ++stats.launches;
+        if (recording != nullptr) recording->push_back({plan.fin_function, 1, plan.block, {buf, buf + n}});
+    }
     ++stats.launches_by_strategy[strategy_name(plan.strategy)];
     for (int o : plan.written_ordinals) {
         DeviceBuffer &d = buffer(ord2base[o]);
@@ -341,6 +360,49 @@ void EngineHip::resolveProfile() {
     pending_events.clear();
 }
 
+std::vector<uint64_t> EngineHip::quietToken() const {
+    return {pool.misses, pool.frees, stats.h2d_bytes, stats.d2h_bytes, scratch_allocs, last_index_checks ? 1u : 0u,
+            flush_cache_misses};
+}
+
+void *EngineHip::graphFor(const std::vector<IssuedLaunch> &launches) {
+    std::string key;
+    for (const IssuedLaunch &l : launches) {
+        key.append(reinterpret_cast<const char *>(&l.function), sizeof(l.function));
+        key.append(reinterpret_cast<const char *>(&l.grid), sizeof(l.grid));
+        key.append(reinterpret_cast<const char *>(&l.block), sizeof(l.block));
+        const uint64_t n = l.args.size();
+        key.append(reinterpret_cast<const char *>(&n), sizeof(n));
+        key.append(reinterpret_cast<const char *>(l.args.data()), l.args.size());
+    }
+    auto it = graph_cache.find(key);
+    if (it != graph_cache.end()) return it->second;
+    if (graph_cache.size() >= 64) {
+        check(bhhip_synchronize(), "synchronize");
+        for (auto &kv : graph_cache) bhhip_graph_destroy(kv.second);
+        graph_cache.clear();
+    }
+    check(bhhip_capture_begin(), "graph capture");
+    for (const IssuedLaunch &l : launches) {
+        if (bhhip_launch(l.function, l.grid, l.block, l.args.data(), l.args.size()) != 0) {
+            void *discard = nullptr;
+            bhhip_capture_end(&discard);
+            throw std::runtime_error(std::string("hip engine: launch during graph capture failed: ") + bhhip_last_error());
+        }
+    }
+    void *exec = nullptr;
+    check(bhhip_capture_end(&exec), "graph instantiation");
+    graph_cache[key] = exec;
+    return exec;
+}
+
+void EngineHip::graphLaunch(void *graph, size_t nlaunches) {
+    check(bhhip_graph_launch(graph), "graph launch");
+    ++graph_replays;
+    stats.launches += nlaunches;
+    ++stats.flushes;
+}
+
 void EngineHip::checkErrors() {
     if (dry_run) return;
     unsigned int flag = 0;
@@ -358,6 +420,8 @@ void EngineHip::handleExecution(BhIR *bhir) {
     std::set<bh_base *> frees;
     std::vector<bh_instruction *> instr_list = jitk::remove_non_computed_system_instr(bhir->instr_list, frees);
     for (bh_base *b : frees) release(b);
+    rewrite_store.clear();
+    if (unflatten) unflatten_full_reductions(instr_list, rewrite_store);
     jitk::set_constructor_flag(instr_list);
 
     auto t = std::chrono::steady_clock::now();
@@ -368,7 +432,7 @@ void EngineHip::handleExecution(BhIR *bhir) {
     static const bool debug = std::getenv("BH_HIP_DEBUG") != nullptr;
     if (hit == flush_cache.end()) {
         ++flush_cache_misses;
-        std::vector<jitk::LoopB> kernels = jitk::get_kernel_list(instr_list, fusion_config);
+        std::vector<jitk::LoopB> kernels = fold_serial_tails(jitk::get_kernel_list(instr_list, fusion_config));
         stats.time_fuse += seconds_since(t);
         t = std::chrono::steady_clock::now();
         CachedFlush cf;
@@ -411,6 +475,7 @@ void EngineHip::handleExecution(BhIR *bhir) {
         for (int o : ck.free_ordinals) release(ord2base[o]);
     }
     const bool index_checks = cf.index_checks;
+    last_index_checks = index_checks;
     if (!cf.reusable) flush_cache.erase(hit);
     if (index_checks) checkErrors();
     for (bh_base *b : bhir->getSyncs()) {
@@ -517,6 +582,7 @@ std::string EngineHip::statistics() const {
        << "\ntime_copy: " << stats.time_copy << "\ntime_total: " << stats.time_total << "\n";
     for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
     ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
+    ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
     ss << "pool_allocated: " << pool.allocated << "\npool_peak: " << pool.peak << "\npool_hits: " << pool.hits
        << "\npool_misses: " << pool.misses << "\n";
     return ss.str();
@@ -573,15 +639,57 @@ public:
     explicit Impl(int stack_level) : ComponentVE(stack_level), engine(*this) {}
     ~Impl() override = default;
 
+    static bool has_slides(const BhIR &bhir) {
+        for (const bh_instruction &i : bhir.instr_list)
+            for (const bh_view &v : i.operand)
+                if (!v.isConstant() && v.hasSlide()) return true;
+        return false;
+    }
+
+    bool has_extmethods(const BhIR &bhir) const {
+        for (const bh_instruction &i : bhir.instr_list)
+            if (extmethods.count(i.opcode) || child_extmethods.count(i.opcode)) return true;
+        return false;
+    }
+
     void execute(BhIR *bhir) override {
         if (disabled) {
             child.execute(bhir);
             return;
         }
         bh_base *cond = bhir->getRepeatCondition();
-        for (uint64_t i = 0; i < bhir->getNRepeats(); ++i) {
-            engine.handleExtmethod(bhir);
-            engine.handleExecution(bhir);
+        const uint64_t nrepeats = bhir->getNRepeats();
+        // A repeat body that only launches kernels, with the same arguments in two successive
+        // iterations, is captured once as a graph and replayed for the remaining iterations.
+        const bool graphable = engine.graph_repeats && !engine.profile && !engine.dry_run && cond == nullptr &&
+                               nrepeats >= 4 && !has_slides(*bhir) && !has_extmethods(*bhir);
+        std::vector<EngineHip::IssuedLaunch> prev, cur;
+        void *graph = nullptr;
+        size_t graph_launches = 0;
+        for (uint64_t i = 0; i < nrepeats; ++i) {
+            if (graph != nullptr) {
+                engine.graphLaunch(graph, graph_launches);
+                continue;
+            }
+            const bool rec = graphable && i >= 1;
+            const std::vector<uint64_t> q0 = engine.quietToken();
+            cur.clear();
+            if (rec) engine.recording = &cur;
+            try {
+                engine.handleExtmethod(bhir);
+                engine.handleExecution(bhir);
+            } catch (...) {
+                engine.recording = nullptr;
+                throw;
+            }
+            engine.recording = nullptr;
+            if (rec) {
+                if (i >= 2 && !cur.empty() && cur == prev && engine.quietToken() == q0) {
+                    graph = engine.graphFor(cur);
+                    graph_launches = cur.size();
+                }
+                prev.swap(cur);
+            }
             if (cond != nullptr) {
                 engine.copyToHost(cond);
                 if (cond->getDataPtr() != nullptr && !static_cast<bool *>(cond->getDataPtr())[0]) break;
@@ -620,6 +728,34 @@ public:
             engine.profile = true;
         } else if (msg == "profile: off") {
             engine.profile = false;
+        } else if (msg.rfind("codegen:", 0) == 0) {
+            // "codegen: max_grid=G map_unroll=U reduce_unroll=V" (any subset); drops cached plans.
+            std::istringstream in(msg.substr(8));
+            std::string kv;
+            while (in >> kv) {
+                const size_t eq = kv.find('=');
+                if (eq == std::string::npos) throw std::invalid_argument("hip engine: bad codegen setting: " + kv);
+                const std::string k = kv.substr(0, eq);
+                const int v = std::stoi(kv.substr(eq + 1));
+                if (v < 1) throw std::invalid_argument("hip engine: codegen setting must be >= 1: " + kv);
+                if (k == "max_grid") {
+                    engine.codegenConfig().max_grid = v;
+                } else if (k == "map_unroll") {
+                    engine.codegenConfig().map_unroll = v;
+                } else if (k == "reduce_unroll") {
+                    engine.codegenConfig().reduce_unroll = v;
+                } else {
+                    throw std::invalid_argument("hip engine: unknown codegen setting: " + k);
+                }
+            }
+            ss << "max_grid=" << engine.codegenConfig().max_grid << " map_unroll=" << engine.codegenConfig().map_unroll
+               << " reduce_unroll=" << engine.codegenConfig().reduce_unroll;
+        } else if (msg == "graph: on") {
+            engine.graph_repeats = true;
+        } else if (msg == "graph: off") {
+            engine.graph_repeats = false;
+        } else if (msg == "graph replays") {
+            ss << engine.graph_replays;
         } else if (msg == "plan log") {
             for (const std::string &l : engine.dry_run_log) ss << l << "\n";
             engine.dry_run_log.clear();

@@ -19,6 +19,7 @@
 #include <bohrium/bh_component.hpp>
 #include <bohrium/jitk/fuser.hpp>
 #include "codegen.hpp"
+#include "rewrite.hpp"
 
 namespace bohrium {
 namespace vehip {
@@ -38,7 +39,7 @@ class DevicePool {
 
 public:
     uint64_t limit = 64ull << 30;
-    uint64_t allocated = 0, peak = 0, hits = 0, misses = 0;
+    uint64_t allocated = 0, peak = 0, hits = 0, misses = 0, frees = 0;
     void *alloc(uint64_t bytes);
     void release(void *ptr, uint64_t bytes);
     void trim(uint64_t keep);
@@ -80,6 +81,32 @@ public:
     /* Resolve the events of profiled launches into last_launches (synchronizes). */
     void resolveProfile();
 
+    /* One launch as issued: kernel, shape and the packed argument bytes. */
+    struct IssuedLaunch {
+        void *function = nullptr;
+        uint64_t grid = 0;
+        uint32_t block = 0;
+        std::vector<uint8_t> args;
+        bool operator==(const IssuedLaunch &o) const {
+            return function == o.function && grid == o.grid && block == o.block && args == o.args;
+        }
+    };
+    /* When set, every launch is also appended here (used to find repeat bodies that can be graph-replayed). */
+    std::vector<IssuedLaunch> *recording = nullptr;
+    /* Changes whenever the engine allocates device memory, copies between host and device or checks
+     * index errors: a repeat iteration whose token is unchanged did nothing but launch kernels. */
+    std::vector<uint64_t> quietToken() const;
+    /* A graph holding `launches` in order (captured once per distinct sequence, then cached). */
+    void *graphFor(const std::vector<IssuedLaunch> &launches);
+    void graphLaunch(void *graph, size_t nlaunches);
+    bool graph_repeats = true;
+    uint64_t graph_replays = 0;
+    /* Kernel-writer settings; changing them through this reference drops every cached plan. */
+    WriterConfig &codegenConfig() {
+        flush_cache.clear();
+        return wcfg;
+    }
+
     jitk::FusionConfig fusion_config;
     bool profile = false;
     /* Compile-only mode: plan and hiprtc-compile every kernel, touch no device. */
@@ -89,6 +116,8 @@ public:
 private:
     component::ComponentVE &comp;
     WriterConfig wcfg;
+    bool unflatten = true;       // restate full reductions of N-D temporaries on their shape
+    RewriteStore rewrite_store;  // instructions and temporaries added to the current flush
     std::string cache_dir;
     bool resident = true;
     std::unordered_map<const bh_base *, DeviceBuffer> bufs;
@@ -111,6 +140,9 @@ private:
     };
     std::unordered_map<std::string, CachedFlush> flush_cache;
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
+    bool last_index_checks = false;
+    uint64_t scratch_allocs = 0;
+    std::map<std::string, void *> graph_cache;
     std::string flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
                          std::unordered_map<const bh_base *, int> &base2ord) const;
     void launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2base,

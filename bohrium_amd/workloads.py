@@ -102,3 +102,16 @@ def jacobi(n, iters):
     G = jacobi_grid(n)
     deltas = [jacobi_step(G) for _ in range(iters)]
     return G, deltas
+
+
+def jacobi_repeat(G, iters):
+    """`iters` Jacobi sweeps of G in place as one recorded body replayed by the engine
+    (do_while); returns the 0-d array with the last sweep's max |new - old|."""
+    delta = _b.empty((), G.dtype)
+
+    def body():
+        delta[...] = jacobi_step(G)
+
+    from . import do_while
+    do_while(body, iters)
+    return delta

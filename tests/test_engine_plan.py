@@ -28,6 +28,12 @@ plan("bs_step", lambda: workloads.black_scholes_step(S, K, T, 3))
 G = workloads.jacobi_grid(64)
 bh.flush(); bh.message("plan log")
 plan("jacobi_step", lambda: workloads.jacobi_step(G))
+G2 = workloads.jacobi_grid(64)
+bh.flush(); bh.message("plan log")
+plan("jacobi_repeat", lambda: workloads.jacobi_repeat(G2, 3))
+S2, K2, T2 = workloads.black_scholes_inputs(1000)
+bh.flush(); bh.message("plan log")
+plan("bs_repeat", lambda: workloads.black_scholes_repeat(S2, K2, T2, 4))
 plan("scan", lambda: (bh.arange(1000) > 5).astype(np.int64).cumsum())
 plan("row_sum", lambda: bh.ones((100, 50)).sum(axis=1))
 plan("col_sum", lambda: bh.ones((100, 50)).sum(axis=0))
@@ -57,9 +63,18 @@ def test_black_scholes_step_is_one_fused_reduction(plans):
 
 
 def test_jacobi_step_plan(plans):
-    # stencil + |tmp - c| (2-d map), the max over the flattened delta, the copy back
-    assert plans["jacobi_step"].count("serial") == 0
-    assert "reduce" in plans["jacobi_step"]
+    """The flattened max over |tmp - c| is restated on the stencil's 2-d shape, so the
+    stencil, the delta and its max are one reduction; the copy back into the grid's
+    interior overlaps the stencil's reads and is a second launch."""
+    assert plans["jacobi_step"] == ["reduce", "map"]
+
+
+def test_repeat_bodies_fold_scalar_tails(plans):
+    """The scalar stores after a reduction (delta[...] = ..., sums[i] = ...) run in the
+    reduction's finalize kernel, so each repeat is one reduce (+ the Jacobi copy)."""
+    assert plans["jacobi_repeat"] == ["reduce", "map"] * 3
+    # (the first launch builds the time offsets before the loop)
+    assert plans["bs_repeat"] == ["map"] + ["reduce"] * 4
 
 
 def test_scan_plan(plans):

@@ -293,3 +293,56 @@ def test_black_scholes_repeat_full_size_sums():
         return workloads.black_scholes_repeat(S, K, T, steps=10)
     got, want = run_both(prog)
     assert_parity(got[0], want[0], np.float64, rtol=1e-12)
+
+
+def _jacobi_repeat_prog(n, iters):
+    def prog():
+        G = workloads.jacobi_grid(n)
+        d = workloads.jacobi_repeat(G, iters)
+        return [G, d]
+    return prog
+
+
+def test_jacobi_repeat_graph_replay_matches_eager_and_oracle():
+    """The repeat body is graph-captured after two identical iterations; the replayed
+    result must equal the eager one bit for bit and the oracle within fp32 tolerance."""
+    bh.message("graph: off")
+    try:
+        eager = [a.asnumpy() for a in _jacobi_repeat_prog(258, 12)()]
+    finally:
+        bh.message("graph: on")
+    before = int(bh.message("graph replays"))
+    got, want = run_both(_jacobi_repeat_prog(258, 12))
+    assert int(bh.message("graph replays")) > before, "the repeat body was not replayed as a graph"
+    for g, e in zip(got, eager):
+        np.testing.assert_array_equal(g, e)
+    for g, w in zip(got, want):
+        assert_parity(g, w, np.float32, rtol=1e-5, atol=1e-3)
+
+
+def test_jacobi_full_grid_20_sweeps():
+    """8192 x 8192 fp32 grid (the benchmark's size), 20 replayed sweeps vs the oracle."""
+    got, want = run_both(_jacobi_repeat_prog(8192, 20))
+    assert_parity(got[1], want[1], np.float32, rtol=1e-5)
+    assert_parity(got[0], want[0], np.float32, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.parametrize("unroll", [1, 2, 3, 4])
+def test_unrolled_grid_stride_parity(unroll):
+    """Unrolled map and reduce skeletons (sizes that leave a remainder) against the oracle."""
+    old = bh.message("codegen:")
+    bh.message("codegen: map_unroll=%d reduce_unroll=%d max_grid=64" % (unroll, unroll))
+    try:
+        def prog():
+            S, K, T = workloads.black_scholes_inputs(64 * 256 * unroll * 3 + 77)
+            x = bh.random.RandomState(3).random_sample((301, 257))
+            y = x[1:, :-1] * 2.0 + x[:-1, 1:]
+            return [workloads.black_scholes_step(S, K, T, 1), y, y.sum(), (y > 1.0).sum(), workloads.mc_pi(100003)]
+        got, want = run_both(prog)
+        assert_parity(got[0], want[0], np.float64, rtol=1e-12)
+        assert_parity(got[1], want[1], np.float64)
+        assert_parity(got[2], want[2], np.float64, rtol=1e-12)
+        assert int(got[3]) == int(want[3])
+        assert int(got[4]) == int(want[4])
+    finally:
+        bh.message("codegen: " + old)
