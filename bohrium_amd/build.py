@@ -31,7 +31,7 @@ CORE = ["core/bh_type.cpp", "core/bh_constant.cpp", "core/bh_view.cpp", "core/bh
 BRIDGE = ["bridge/runtime.cpp"]
 NODE = ["vem_node/node.cpp"]
 VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp"]
-VE_HIP_HIP = ["ve_hip/bhhip.cpp", "ve_hip/engine.cpp"]
+VE_HIP_HIP = ["ve_hip/bhhip.cpp", "ve_hip/engine.cpp", "ve_hip/batch.cpp"]
 
 
 def _run(cmd):

@@ -31,6 +31,9 @@ struct FusionConfig {
     MergeWeight learned_weight;
     /* Engine-specific veto on merging two blocks (e.g. keep root blocks on a parallel skeleton). */
     std::function<bool(const LoopB &, const LoopB &)> merge_filter;
+    /* Keep an accumulate out of the pre-fuser's groups that hold any other sweep (a GPU scan
+     * skeleton cannot also reduce); the greedy fuser applies merge_filter instead. */
+    bool accumulate_alone = false;
     bool can_merge(const Block &a, const Block &b) const {
         if (!mergeable(a, b, avoid_rank0_sweep)) return false;
         return !merge_filter || a.isInstr() || b.isInstr() || merge_filter(a.getLoop(), b.getLoop());
@@ -39,7 +42,7 @@ struct FusionConfig {
 
 std::vector<InstrPtr> simplify_instr_list(const std::vector<bh_instruction *> &instr_list);
 void simplify_instr(bh_instruction &instr);
-std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list);
+std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone = false);
 std::vector<Block> fuser_singleton(const std::vector<bh_instruction *> &instr_list);
 void fuser_serial(std::vector<Block> &block_list, const FusionConfig &config);
 void fuser_greedy(const FusionConfig &config, std::vector<Block> &block_list, const MergeWeight &weight_fn);

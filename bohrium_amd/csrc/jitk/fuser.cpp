@@ -110,7 +110,7 @@ std::vector<InstrPtr> simplify_instr_list(const std::vector<bh_instruction *> &i
     return ret;
 }
 
-std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list) {
+std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone) {
     const std::vector<InstrPtr> list = simplify_instr_list(instr_list);
     std::vector<std::vector<InstrPtr> > groups;
     for (auto it = list.begin(); it != list.end();) {
@@ -121,10 +121,51 @@ std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_li
             ++it;
             continue;
         }
+        bool has_acc = bh_opcode_is_accumulate((*it)->opcode) != 0;
+        bool has_sweep = bh_opcode_is_sweep((*it)->opcode) != 0;
         ++it;
         for (; it != list.end(); ++it) {
+            const bool acc = bh_opcode_is_accumulate((*it)->opcode) != 0;
+            const bool sweep = bh_opcode_is_sweep((*it)->opcode) != 0;
+            if (accumulate_alone && ((has_acc && sweep) || (acc && has_sweep))) {
+                if (acc) {
+                    // The element-wise instructions at the end of the group that only feed this
+                    // accumulate start its group instead, so their results need no array.
+                    std::set<const bh_base *> feeds;
+                    for (size_t o = 1; o < (*it)->operand.size(); ++o)
+                        if (!(*it)->operand[o].isConstant()) feeds.insert((*it)->operand[o].base);
+                    size_t keep = g.size();
+                    while (keep > 1) {
+                        const InstrPtr &c = g[keep - 1];
+                        if (bh_opcode_is_system(c->opcode)) {
+                            --keep;
+                            continue;
+                        }
+                        if (bh_opcode_is_sweep(c->opcode) || c->operand.empty() || !feeds.count(c->operand[0].base)) break;
+                        for (size_t o = 1; o < c->operand.size(); ++o)
+                            if (!c->operand[o].isConstant()) feeds.insert(c->operand[o].base);
+                        --keep;
+                    }
+                    while (keep < g.size() && bh_opcode_is_system(g[keep]->opcode)) ++keep;  // frees stay put
+                    if (keep < g.size()) {
+                        std::vector<InstrPtr> moved(g.begin() + static_cast<std::ptrdiff_t>(keep), g.end());
+                        g.resize(keep);
+                        groups.push_back(moved);
+                        std::vector<InstrPtr> &n = groups.back();
+                        n.push_back(*it);
+                        ++it;
+                        for (; it != list.end(); ++it) {
+                            if (bh_opcode_is_sweep((*it)->opcode) || !fully_fusible(n, *it)) break;
+                            n.push_back(*it);
+                        }
+                    }
+                }
+                break;
+            }
             if (fully_fusible(g, *it)) {
                 g.push_back(*it);
+                has_acc |= acc;
+                has_sweep |= sweep;
             } else {
                 break;
             }
@@ -587,7 +628,7 @@ std::vector<Block> get_block_list(const std::vector<bh_instruction *> &instr_lis
     if (config.pre_fuser == "none" || config.pre_fuser == "singleton") {
         block_list = fuser_singleton(instr_list);
     } else if (config.pre_fuser == "lossy" || config.pre_fuser == "pre_fuser_lossy") {
-        block_list = pre_fuser_lossy(instr_list);
+        block_list = pre_fuser_lossy(instr_list, config.accumulate_alone);
     } else {
         throw std::runtime_error("Unknown pre-fuser: " + config.pre_fuser);
     }

@@ -139,6 +139,12 @@ int bhhip_memcpy_d2d(void *dst, const void *src, size_t bytes) {
     return 0;
 }
 
+int bhhip_memcpy_h2d_async(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream), "hipMemcpyAsync(H2D)");
+    return 0;
+}
+
 int bhhip_memzero(void *dptr, size_t bytes) {
     if (bytes == 0) return 0;
     HIPCHK(hipMemsetAsync(dptr, 0, bytes, g_stream), "hipMemsetAsync");
@@ -285,6 +291,11 @@ int bhhip_event_elapsed_ms(void *start, void *stop, float *ms) {
     HIPCHK(hipEventSynchronize(static_cast<hipEvent_t>(stop)), "hipEventSynchronize");
     HIPCHK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)),
            "hipEventElapsedTime");
+    return 0;
+}
+
+int bhhip_event_synchronize(void *event) {
+    HIPCHK(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
     return 0;
 }
 

@@ -333,6 +333,12 @@ void indent(std::ostringstream &o, int n) { o << std::string(n * 2, ' '); }
 
 // ------------------------------------------------------------------ writer
 
+struct KernelParam {
+    std::string type;  // value type: "double *", "int64_t", ...
+    std::string name;
+    bool is_ptr;
+};
+
 class Writer {
 public:
     Writer(const LoopB &unit, const WriterConfig &cfg) : unit(unit), cfg(cfg) { collect(); }
@@ -451,7 +457,7 @@ private:
     void emit_full_reduce();
     void emit_scan();
     void emit_serial();
-    std::string signature(const std::string &extra);
+    std::vector<KernelParam> parameters();
 };
 
 void Writer::collect() {
@@ -635,11 +641,10 @@ void Writer::emit_instr(const bh_instruction &instr, Scope &sc, int ind) {
     body << hip_operation(instr, ops) << "\n";
 }
 
-std::string Writer::signature(const std::string &extra) {
-    std::ostringstream s;
-    std::vector<std::string> params;
-    auto lit = [&](const std::string &decl, const void *p, int size, int align) {
-        params.push_back(decl);
+std::vector<KernelParam> Writer::parameters() {
+    std::vector<KernelParam> params;
+    auto lit = [&](const std::string &type, const std::string &name, const void *p, int size, int align) {
+        params.push_back({type, name, false});
         ArgSpec a;
         a.kind = ArgSpec::LITERAL;
         a.bytes.assign(static_cast<const uint8_t *>(p), static_cast<const uint8_t *>(p) + size);
@@ -648,7 +653,7 @@ std::string Writer::signature(const std::string &extra) {
         kp.args.push_back(a);
     };
     for (bh_base *b : kp.params) {
-        params.push_back(std::string(hip_type(b->dtype())) + " *__restrict__ a" + std::to_string(bid.at(b)));
+        params.push_back({std::string(hip_type(b->dtype())) + " *", "a" + std::to_string(bid.at(b)), true});
         ArgSpec a;
         a.kind = ArgSpec::BASE_PTR;
         a.base = b;
@@ -656,7 +661,7 @@ std::string Writer::signature(const std::string &extra) {
     }
     for (size_t k = 0; k < os_views.size(); ++k) {
         const bh_view &v = *os_views[k];
-        lit("const int64_t vo" + std::to_string(k), &v.start, 8, 8);
+        lit("int64_t", "vo" + std::to_string(k), &v.start, 8, 8);
         auto src = sliding_src.find(&v);
         if (src != sliding_src.end()) {
             kp.args.back().view_origin = src->second.first;
@@ -665,41 +670,37 @@ std::string Writer::signature(const std::string &extra) {
             kp.reusable = false;
         }
         for (int64_t d = 0; d < v.ndim; ++d)
-            lit("const int64_t vs" + std::to_string(k) + "_" + std::to_string(d), &v.stride[d], 8, 8);
+            lit("int64_t", "vs" + std::to_string(k) + "_" + std::to_string(d), &v.stride[d], 8, 8);
     }
-    for (size_t j = 0; j < loops.size(); ++j) lit("const int64_t n" + std::to_string(j), &loops[j]->size, 8, 8);
+    for (size_t j = 0; j < loops.size(); ++j) lit("int64_t", "n" + std::to_string(j), &loops[j]->size, 8, 8);
     for (const bh_base *b : index_checked) {
         const int64_t n = b->nelem();
-        lit("const int64_t nb" + std::to_string(bid.at(b)), &n, 8, 8);
+        lit("int64_t", "nb" + std::to_string(bid.at(b)), &n, 8, 8);
     }
     for (size_t j = 0; j < const_instrs.size(); ++j) {
         const bh_constant &c = const_instrs[j]->constant;
         const std::string cn = "c" + std::to_string(j);
         if (c.type == bh_type::R123) {
-            lit("const uint64_t " + cn + "_start", &c.value.r123.start, 8, 8);
+            lit("uint64_t", cn + "_start", &c.value.r123.start, 8, 8);
             kp.args.back().const_origin = const_instrs[j]->origin_id;
             kp.args.back().const_part = 0;
-            lit("const uint64_t " + cn + "_key", &c.value.r123.key, 8, 8);
+            lit("uint64_t", cn + "_key", &c.value.r123.key, 8, 8);
             kp.args.back().const_origin = const_instrs[j]->origin_id;
             kp.args.back().const_part = 1;
         } else {
             const int size = bh_type_size(c.type);
             const int align = bh_type_is_complex(c.type) ? size / 2 : size;
-            lit("const " + std::string(hip_type(c.type)) + " " + cn, &c.value, size, align);
+            lit(hip_type(c.type), cn, &c.value, size, align);
             kp.args.back().const_origin = const_instrs[j]->origin_id;
         }
     }
     if (kp.uses_index_check) {
-        params.push_back("unsigned int *__restrict__ bh_err");
+        params.push_back({"unsigned int *", "bh_err", true});
         ArgSpec a;
         a.kind = ArgSpec::ERRFLAG_PTR;
         kp.args.push_back(a);
     }
-    s << "extern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.name << "(";
-    for (size_t i = 0; i < params.size(); ++i) s << (i ? ",\n    " : "") << params[i];
-    if (!extra.empty()) s << (params.empty() ? "" : ",\n    ") << extra;
-    s << ")";
-    return s.str();
+    return params;
 }
 
 // ---- MAP: grid-stride over the flattened leading parallel ranks
@@ -729,8 +730,8 @@ std::vector<const bh_view *> Writer::preloadable(const LoopB &l) {
 // into registers and then compute them, so a wave has all of its loads in flight at once;
 // the remaining elements run one at a time.
 void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
-    body << "  const int64_t st_ = (int64_t)gridDim.x * BH_WG;\n";
-    body << "  int64_t g_ = (int64_t)blockIdx.x * BH_WG + threadIdx.x;\n";
+    body << "  const int64_t st_ = (int64_t)bh_nblk * BH_WG;\n";
+    body << "  int64_t g_ = (int64_t)bh_bid * BH_WG + threadIdx.x;\n";
     if (unroll > 1) {
         const std::vector<const bh_view *> reads =
             e.body_loop != nullptr ? preloadable(*e.body_loop) : std::vector<const bh_view *>();
@@ -910,11 +911,9 @@ void Writer::emit_full_reduce() {
         InstrPtr instr;
         std::string var;
         bh_type t;
-        int64_t part_off;
         bool is_tmp;
     };
     std::vector<Acc> accs;
-    int64_t off = 0;
     const std::set<bh_base *> ktmps = unit.getLocalTemps();
     for (const InstrPtr &s : sweeps) {
         const bh_view &v = s->operand[0];
@@ -937,11 +936,19 @@ void Writer::emit_full_reduce() {
                 body << "  " << hip_type(t) << " " << a.var << " = " << ident << ";\n";
             }
         }
-        a.part_off = off;
-        off += ((static_cast<int64_t>(grid) * bh_type_size(t) + 255) / 256) * 256;
         accs.push_back(a);
     }
-    kp.scratch_bytes = off;
+    // Partials: one 16-byte slot per workgroup and distinct accumulator, workgroup-major, so the
+    // layout (and the kernel source) does not depend on the grid size.
+    std::map<std::string, int> slot;
+    for (const Acc &a : accs)
+        if (!slot.count(a.var)) slot.emplace(a.var, static_cast<int>(slot.size()));
+    const int nslots = static_cast<int>(slot.size());
+    kp.scratch_bytes = static_cast<int64_t>(grid) * nslots * 16;
+    auto slot_addr = [&](const Acc &a, const std::string &wg) {
+        return "(bh_scratch + ((size_t)" + wg + " * " + std::to_string(nslots) + " + " +
+               std::to_string(slot.at(a.var)) + ") * 16)";
+    };
     declare_scope(unit, ksc, 1);
     if (!pairs.empty()) {
         // One flat grid-stride loop over (i0, i1); each row temporary is the accumulator
@@ -1000,8 +1007,7 @@ void Writer::emit_full_reduce() {
         const std::string T = hip_type(a.t);
         body << "  { " << op_lambda(a) << "\n";
         body << "    " << a.var << " = bh_block_reduce(" << a.var << ", op_, (" << T << " *)bh_lds_);\n";
-        body << "    if (threadIdx.x == 0) ((" << T << " *)(bh_scratch + " << a.part_off << "))[blockIdx.x] = " << a.var
-             << ";\n";
+        body << "    if (threadIdx.x == 0) *(" << T << " *)" << slot_addr(a, "bh_bid") << " = " << a.var << ";\n";
         body << "    __syncthreads(); }\n";
     }
     main_body = body.str();
@@ -1023,9 +1029,9 @@ void Writer::emit_full_reduce() {
         }
         body << "  " << T << " " << a.var << ";\n";
         body << "  { " << op_lambda(a) << "\n";
-        body << "    const " << T << " *p_ = (const " << T << " *)(bh_scratch + " << a.part_off << ");\n";
         body << "    " << T << " v_ = " << literal(jitk::sweep_identity(a.instr->opcode, a.t)) << ";\n";
-        body << "    for (unsigned j_ = threadIdx.x; j_ < " << grid << "u; j_ += BH_WG) v_ = op_(v_, p_[j_]);\n";
+        body << "    for (unsigned j_ = threadIdx.x; j_ < bh_nparts; j_ += BH_WG) v_ = op_(v_, *(const " << T << " *)"
+             << slot_addr(a, "j_") << ");\n";
         body << "    " << a.var << " = bh_block_reduce(v_, op_, (" << T << " *)bh_lds_);\n";
         body << "    __syncthreads(); }\n";
     }
@@ -1055,11 +1061,9 @@ void Writer::emit_scan() {
     const int64_t tile_elems = static_cast<int64_t>(kp.block) * items;
     const int64_t ntiles = (l0.size + tile_elems - 1) / tile_elems;
     kp.grid = static_cast<uint64_t>(std::max<int64_t>(1, ntiles));
-    // Scratch layout: [epoch u32 | tile counter u32 | pad to 256] flags[ntiles] u32, agg[ntiles] T, inc[ntiles] T
-    const int64_t flags_off = 256;
-    const int64_t agg_off = flags_off + ((ntiles * 4 + 255) / 256) * 256;
-    const int64_t inc_off = agg_off + ((ntiles * bh_type_size(t) + 255) / 256) * 256;
-    kp.scratch_bytes = inc_off + ((ntiles * bh_type_size(t) + 255) / 256) * 256;
+    // Scratch layout: [epoch u32 | tile counter u32 | pad to 256], then one 48-byte record per tile
+    // (flag u32 at +0, aggregate at +16, inclusive prefix at +32); no offset depends on the size.
+    kp.scratch_bytes = 256 + ntiles * 48;
     const std::string ident = literal(jitk::sweep_identity(acc->opcode, t));
     scan_instr = acc.get();
     scan_sink = "x_";
@@ -1068,9 +1072,10 @@ void Writer::emit_scan() {
          << combine_expr(acc->opcode, "x", "y") << "; };\n";
     body << "  unsigned int *bh_epoch_p = (unsigned int *)bh_scratch;\n";
     body << "  unsigned int *bh_tiles_p = bh_epoch_p + 1;\n";
-    body << "  unsigned int *bh_flag = (unsigned int *)(bh_scratch + " << flags_off << ");\n";
-    body << "  " << T << " *bh_agg = (" << T << " *)(bh_scratch + " << agg_off << ");\n";
-    body << "  " << T << " *bh_inc = (" << T << " *)(bh_scratch + " << inc_off << ");\n";
+    body << "  unsigned char *bh_rec = bh_scratch + 256;\n";
+    body << "#define BH_FLAG(j) ((unsigned int *)(bh_rec + (size_t)(j) * 48))\n";
+    body << "#define BH_AGG(j) (*(" << T << " *)(bh_rec + (size_t)(j) * 48 + 16))\n";
+    body << "#define BH_INC(j) (*(" << T << " *)(bh_rec + (size_t)(j) * 48 + 32))\n";
     body << "  __shared__ unsigned int bh_tile_s, bh_ep_s;\n";
     body << "  __shared__ " << T << " bh_wtot_[BH_WAVES];\n";
     body << "  __shared__ " << T << " bh_excl_s;\n";
@@ -1106,19 +1111,19 @@ void Writer::emit_scan() {
     body << "  if (threadIdx.x == 0) {\n";
     body << "    " << T << " ex_ = " << ident << ";\n";
     body << "    if (tile_ > 0) {\n";
-    body << "      bh_agg[tile_] = ttot_;\n";
-    body << "      __hip_atomic_store(&bh_flag[tile_], ep_ + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "      BH_AGG(tile_) = ttot_;\n";
+    body << "      __hip_atomic_store(BH_FLAG(tile_), ep_ + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
     body << "      for (int64_t j_ = tile_ - 1;; --j_) {\n";
     body << "        unsigned int f_;\n";
-    body << "        do { f_ = __hip_atomic_load(&bh_flag[j_], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }"
+    body << "        do { f_ = __hip_atomic_load(BH_FLAG(j_), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }"
             " while (f_ != ep_ + 1u && f_ != ep_ + 2u);\n";
-    body << "        if (f_ == ep_ + 2u) { ex_ = op_(bh_inc[j_], ex_); break; }\n";
-    body << "        ex_ = op_(bh_agg[j_], ex_);\n";
+    body << "        if (f_ == ep_ + 2u) { ex_ = op_(BH_INC(j_), ex_); break; }\n";
+    body << "        ex_ = op_(BH_AGG(j_), ex_);\n";
     body << "      }\n    }\n";
-    body << "    bh_inc[tile_] = op_(ex_, ttot_);\n";
-    body << "    __hip_atomic_store(&bh_flag[tile_], ep_ + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "    BH_INC(tile_) = op_(ex_, ttot_);\n";
+    body << "    __hip_atomic_store(BH_FLAG(tile_), ep_ + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
     body << "    bh_excl_s = ex_;\n";
-    body << "    if (tile_ == (int64_t)gridDim.x - 1) {\n";
+    body << "    if (tile_ == (int64_t)bh_nblk - 1) {\n";
     body << "      *bh_tiles_p = 0u;\n";
     body << "      __hip_atomic_store(bh_epoch_p, bh_ep_s + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
     body << "    }\n  }\n";
@@ -1129,6 +1134,7 @@ void Writer::emit_scan() {
     body << "    const int64_t i0 = tb_ + ((int64_t)wave_ * " << items << " + j_) * 64 + lane_;\n";
     body << "    if (i0 < n" << lid.at(&l0) << ") a" << bid.at(out.base) << "[" << index(out) << "] = op_(pre_, v_[j_]);\n";
     body << "  }\n";
+    body << "#undef BH_FLAG\n#undef BH_AGG\n#undef BH_INC\n";
     write_back(unit, ksc, 1);
 }
 
@@ -1136,7 +1142,7 @@ void Writer::emit_scan() {
 void Writer::emit_serial() {
     kp.block = 64;
     kp.grid = 1;
-    body << "  if (blockIdx.x != 0 || threadIdx.x != 0) return;\n";
+    body << "  if (bh_bid != 0 || threadIdx.x != 0) return;\n";
     emit_body(unit, nullptr, 1);
     kp.work_items = 1;
 }
@@ -1151,28 +1157,51 @@ KernelPlan Writer::plan(Strategy s) {
         case Strategy::SCAN: emit_scan(); break;
         case Strategy::SERIAL: emit_serial(); break;
     }
-    std::string extra;
-    if (s == Strategy::FULL_REDUCE || s == Strategy::SCAN) extra = "unsigned char *__restrict__ bh_scratch";
+    std::vector<KernelParam> params = parameters();
+    if (s == Strategy::FULL_REDUCE || s == Strategy::SCAN) {
+        params.push_back({"unsigned char *", "bh_scratch", true});
+        ArgSpec a;
+        a.kind = ArgSpec::SCRATCH_PTR;
+        kp.args.push_back(a);
+    }
+    if (s == Strategy::FULL_REDUCE) {
+        // The finalize kernel folds this many partials (the main kernel's grid).
+        params.push_back({"uint32_t", "bh_nparts", false});
+        const uint32_t n = static_cast<uint32_t>(kp.grid);
+        ArgSpec a;
+        a.kind = ArgSpec::LITERAL;
+        a.bytes.assign(reinterpret_cast<const uint8_t *>(&n), reinterpret_cast<const uint8_t *>(&n) + 4);
+        a.size = 4;
+        a.align = 4;
+        kp.args.push_back(a);
+    }
+    kp.param_decls.clear();
+    kp.param_types.clear();
+    kp.param_names.clear();
+    for (const KernelParam &p : params) {
+        kp.param_decls.push_back(p.is_ptr ? p.type + "__restrict__ " + p.name : "const " + p.type + " " + p.name);
+        kp.param_types.push_back(p.type);
+        kp.param_names.push_back(p.name);
+    }
+    // The body is a device function of the parameters and of the workgroup's place in its
+    // grid; the kernels below (and the batched kernels of batch_source) only call it.
+    std::string plist, nlist;
+    for (size_t i = 0; i < params.size(); ++i) {
+        plist += (i ? ",\n    " : "") + kp.param_decls[i];
+        nlist += (i ? ", " : "") + params[i].name;
+    }
+    std::ostringstream dev;
+    const std::string main_src = s == Strategy::FULL_REDUCE ? main_body : body.str();
+    dev << "__device__ __forceinline__ void bh_main_(" << plist << (plist.empty() ? "" : ",\n    ")
+        << "const unsigned bh_bid, const unsigned bh_nblk) {\n" << main_src << "}\n";
+    if (s == Strategy::FULL_REDUCE) dev << "\n__device__ __forceinline__ void bh_fin_(" << plist << ") {\n" << body.str() << "}\n";
+    kp.device_code = dev.str();
     std::ostringstream src;
-    const std::string sig = signature(extra);
-    if (s == Strategy::FULL_REDUCE) {
-        // Both kernels take the same arguments, so one packed buffer serves both launches.
-        std::string fsig = sig;
-        const std::string from = " " + kp.name + "(";
-        fsig.replace(fsig.find(from), from.size(), " " + kp.fin_name + "(");
-        src << sig << " {\n" << main_body << "}\n\n" << fsig << " {\n" << body.str() << "}\n";
-    } else {
-        src << sig << " {\n" << body.str() << "}\n";
-    }
-    if (s == Strategy::FULL_REDUCE) {
-        ArgSpec a;
-        a.kind = ArgSpec::SCRATCH_PTR;
-        kp.args.push_back(a);
-    } else if (s == Strategy::SCAN) {
-        ArgSpec a;
-        a.kind = ArgSpec::SCRATCH_PTR;
-        kp.args.push_back(a);
-    }
+    src << kp.device_code << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.name << "("
+        << plist << ") {\n  bh_main_(" << nlist << (nlist.empty() ? "" : ", ") << "blockIdx.x, gridDim.x);\n}\n";
+    if (s == Strategy::FULL_REDUCE)
+        src << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.fin_name << "(" << plist
+            << ") {\n  bh_fin_(" << nlist << ");\n}\n";
     kp.source = src.str();
     return kp;
 }
@@ -1378,6 +1407,50 @@ std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg
         }
     }
     return ret;
+}
+
+std::vector<size_t> arg_offsets(const std::vector<ArgSpec> &args, size_t *total) {
+    std::vector<size_t> off;
+    size_t n = 0;
+    for (const ArgSpec &a : args) {
+        while (n % static_cast<size_t>(a.align)) ++n;
+        off.push_back(n);
+        n += static_cast<size_t>(a.size);
+    }
+    while (n % 8) ++n;
+    if (total != nullptr) *total = n;
+    return off;
+}
+
+std::string batch_source(const KernelPlan &p) {
+    const std::vector<size_t> off = arg_offsets(p.args, nullptr);
+    std::string unpack;
+    for (size_t i = 0; i < p.param_types.size(); ++i) {
+        const std::string &t = p.param_types[i];
+        const bool ptr = !t.empty() && t.back() == '*';
+        unpack += std::string(i ? ",\n      " : "") + "*(" + t + (ptr ? "const *" : " const *") + ")(A_ + " +
+                  std::to_string(off[i]) + ")";
+    }
+    const std::string sep = unpack.empty() ? "" : ", ";
+    std::ostringstream s;
+    s << p.device_code;
+    // Workgroup b of the batched grid runs workgroup b - first[i] of item i, where
+    // first[i] <= b < first[i + 1]; each item's packed arguments sit at args + i * stride.
+    s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ") bh_kernel_batch("
+      << "const unsigned char *__restrict__ bh_args, const unsigned *__restrict__ bh_first, const unsigned bh_items, "
+         "const unsigned bh_stride) {\n"
+      << "  unsigned lo_ = 0, hi_ = bh_items;\n"
+      << "  while (hi_ - lo_ > 1u) { const unsigned mid_ = (lo_ + hi_) >> 1; if (bh_first[mid_] <= blockIdx.x) lo_ = mid_; "
+         "else hi_ = mid_; }\n"
+      << "  const unsigned char *A_ = bh_args + (size_t)lo_ * bh_stride;\n"
+      << "  bh_main_(" << unpack << sep << "blockIdx.x - bh_first[lo_], bh_first[lo_ + 1] - bh_first[lo_]);\n}\n";
+    if (!p.fin_name.empty()) {
+        s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ") bh_kernel_fin_batch("
+          << "const unsigned char *__restrict__ bh_args, const unsigned bh_stride) {\n"
+          << "  const unsigned char *A_ = bh_args + (size_t)blockIdx.x * bh_stride;\n"
+          << "  bh_fin_(" << unpack << ");\n}\n";
+    }
+    return s.str();
 }
 
 } // namespace vehip

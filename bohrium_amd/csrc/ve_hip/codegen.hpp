@@ -58,6 +58,10 @@ struct KernelPlan {
      * after `name` with the same arguments (folds the per-workgroup partials). */
     std::string fin_name;
     std::vector<ArgSpec> args;
+    /* The body as device functions (bh_main_, and bh_fin_ for FULL_REDUCE) and their parameters,
+     * one per ArgSpec: declaration, value type and name.  `source` adds the kernels calling them. */
+    std::string device_code;
+    std::vector<std::string> param_decls, param_types, param_names;
     uint64_t grid = 1;
     uint32_t block = 256;
     /* Bases that must be resident on the device for this launch, and those it writes. */
@@ -97,6 +101,15 @@ Strategy classify_root_block(const jitk::LoopB &l0);
 
 /* True when merging two root blocks keeps the result on a parallel skeleton. */
 bool root_merge_allowed(const jitk::LoopB &l1, const jitk::LoopB &l2);
+
+/* Byte offset of each argument in the packed kernel-argument buffer (natural alignment),
+ * and the buffer's total size (a multiple of 8). */
+std::vector<size_t> arg_offsets(const std::vector<ArgSpec> &args, size_t *total);
+
+/* Source of the batched form of `plan`: bh_kernel_batch runs many items that share the plan's
+ * code, each with its own packed arguments and workgroup count (and bh_kernel_fin_batch, one
+ * workgroup per item, for FULL_REDUCE). */
+std::string batch_source(const KernelPlan &plan);
 
 /* HIP spelling of an element type inside generated kernels. */
 const char *hip_type(bh_type t);

@@ -50,8 +50,11 @@ std::string default_cache_dir() {
 
 void *DevicePool::alloc(uint64_t bytes) {
     bytes = std::max<uint64_t>(256, (bytes + 255) / 256 * 256);
-    auto it = _free.find(bytes);
-    if (it != _free.end()) {
+    // Last released first: a loop that frees and re-creates a temporary gets the same buffer
+    // every iteration, so its launches (and a graph of them) stay identical.
+    auto range = _free.equal_range(bytes);
+    if (range.first != range.second) {
+        auto it = std::prev(range.second);
         void *p = it->second;
         _free.erase(it);
         _cached -= bytes;
@@ -110,6 +113,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     fusion_config.greedy_threshold = cfg.defaultGet<uint64_t>("greedy_threshold", 10000);
     fusion_config.avoid_rank0_sweep = false;
     fusion_config.merge_filter = [](const jitk::LoopB &a, const jitk::LoopB &b) { return root_merge_allowed(a, b); };
+    fusion_config.accumulate_alone = true;
     wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
     wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
     wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
@@ -117,6 +121,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     unflatten = cfg.defaultGet<bool>("unflatten_reductions", true);
     resident = cfg.defaultGet<bool>("resident", true);
     graph_repeats = cfg.defaultGet<bool>("graph_repeats", true);
+    batch_min = cfg.defaultGet<int>("batch_min", 8);
     profile = cfg.defaultGet<bool>("prof", false);
     cache_dir = cfg.defaultGet<std::string>("cache_dir", "");
     if (cache_dir.empty() || cache_dir == "NONE") cache_dir = default_cache_dir();
@@ -228,53 +233,16 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
         return;
     }
     if (plan.function == nullptr) {
-        check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &plan.function),
-              "kernel compilation");
-        if (!plan.fin_name.empty())
-            check(bhhip_compile(kPrelude, plan.source.c_str(), plan.fin_name.c_str(), cache_dir.c_str(),
-                                &plan.fin_function),
-                  "kernel compilation");
+        compilePlan(plan);
         stats.time_compile += seconds_since(t);
         t = std::chrono::steady_clock::now();
     }
     for (int o : plan.param_ordinals) ensureResident(ord2base[o]);
     if (plan.strategy == Strategy::FULL_REDUCE) ensureScratch(reduce_scratch, reduce_scratch_bytes, plan.scratch_bytes, false);
     if (plan.strategy == Strategy::SCAN) ensureScratch(scan_scratch, scan_scratch_bytes, plan.scratch_bytes, true);
-
     uint8_t buf[4096];
-    size_t n = 0;
-    auto put = [&](const void *p, int size, int align) {
-        while (n % static_cast<size_t>(align)) buf[n++] = 0;
-        if (n + static_cast<size_t>(size) > sizeof(buf)) throw std::runtime_error("hip engine: kernel arguments exceed 4 KiB");
-        std::memcpy(buf + n, p, static_cast<size_t>(size));
-        n += static_cast<size_t>(size);
-    };
-    for (const ArgSpec &a : plan.args) {
-        void *ptr = nullptr;
-        switch (a.kind) {
-            case ArgSpec::LITERAL:
-                if (a.view_origin >= 0 && a.view_origin < static_cast<int64_t>(instrs.size())) {
-                    put(&instrs[a.view_origin]->operand[a.view_operand].start, 8, 8);
-                } else if (a.const_origin >= 0 && a.const_origin < static_cast<int64_t>(instrs.size())) {
-                    const bh_constant &c = instrs[a.const_origin]->constant;
-                    const void *src = a.const_part == 0 ? static_cast<const void *>(&c.value.r123.start)
-                                    : a.const_part == 1 ? static_cast<const void *>(&c.value.r123.key)
-                                                        : static_cast<const void *>(&c.value);
-                    put(src, a.size, a.align);
-                } else {
-                    put(a.bytes.data(), a.size, a.align);
-                }
-                continue;
-            case ArgSpec::BASE_PTR: ptr = bufs.at(ord2base[a.base_ordinal]).ptr; break;
-            case ArgSpec::SCRATCH_PTR:
-                ptr = plan.strategy == Strategy::SCAN ? scan_scratch : reduce_scratch;
-                break;
-            case ArgSpec::COUNTER_PTR: ptr = counters; break;
-            case ArgSpec::ERRFLAG_PTR: ptr = counters + 1; break;
-        }
-        put(&ptr, 8, 8);
-    }
-    while (n % 8) buf[n++] = 0;
+    const size_t n = packArgs(plan, ord2base, instrs, plan.strategy == Strategy::SCAN ? scan_scratch : reduce_scratch,
+                              buf, sizeof(buf));
 
     void *ev0 = nullptr, *ev1 = nullptr;
     if (profile) {
@@ -303,6 +271,56 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
         pending_events.push_back({ev0, ev1, {strategy_name(plan.strategy), plan.work_items, plan.grid, 0.0, hex}});
     }
     stats.time_launch += seconds_since(t);
+}
+
+void EngineHip::compilePlan(KernelPlan &plan) {
+    check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &plan.function),
+          "kernel compilation");
+    if (!plan.fin_name.empty())
+        check(bhhip_compile(kPrelude, plan.source.c_str(), plan.fin_name.c_str(), cache_dir.c_str(), &plan.fin_function),
+              "kernel compilation");
+}
+
+size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
+                           const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap) {
+    size_t n = 0;
+    auto put = [&](const void *p, int size, int align) {
+        while (n % static_cast<size_t>(align)) {
+            if (n >= cap) throw std::runtime_error("hip engine: kernel arguments exceed their buffer");
+            out[n++] = 0;
+        }
+        if (n + static_cast<size_t>(size) > cap) throw std::runtime_error("hip engine: kernel arguments exceed their buffer");
+        std::memcpy(out + n, p, static_cast<size_t>(size));
+        n += static_cast<size_t>(size);
+    };
+    for (const ArgSpec &a : plan.args) {
+        void *ptr = nullptr;
+        switch (a.kind) {
+            case ArgSpec::LITERAL:
+                if (a.view_origin >= 0 && a.view_origin < static_cast<int64_t>(instrs.size())) {
+                    put(&instrs[a.view_origin]->operand[a.view_operand].start, 8, 8);
+                } else if (a.const_origin >= 0 && a.const_origin < static_cast<int64_t>(instrs.size())) {
+                    const bh_constant &c = instrs[a.const_origin]->constant;
+                    const void *src = a.const_part == 0 ? static_cast<const void *>(&c.value.r123.start)
+                                    : a.const_part == 1 ? static_cast<const void *>(&c.value.r123.key)
+                                                        : static_cast<const void *>(&c.value);
+                    put(src, a.size, a.align);
+                } else {
+                    put(a.bytes.data(), a.size, a.align);
+                }
+                continue;
+            case ArgSpec::BASE_PTR: ptr = bufs.at(ord2base[a.base_ordinal]).ptr; break;
+            case ArgSpec::SCRATCH_PTR: ptr = scratch; break;
+            case ArgSpec::COUNTER_PTR: ptr = counters; break;
+            case ArgSpec::ERRFLAG_PTR: ptr = counters + 1; break;
+        }
+        put(&ptr, 8, 8);
+    }
+    while (n % 8) {
+        if (n >= cap) throw std::runtime_error("hip engine: kernel arguments exceed their buffer");
+        out[n++] = 0;
+    }
+    return n;
 }
 
 std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
@@ -362,7 +380,7 @@ void EngineHip::resolveProfile() {
 
 std::vector<uint64_t> EngineHip::quietToken() const {
     return {pool.misses, pool.frees, stats.h2d_bytes, stats.d2h_bytes, scratch_allocs, last_index_checks ? 1u : 0u,
-            flush_cache_misses};
+            flush_cache_misses, batch_launches};
 }
 
 void *EngineHip::graphFor(const std::vector<IssuedLaunch> &launches) {
@@ -469,10 +487,15 @@ void EngineHip::handleExecution(BhIR *bhir) {
         stats.time_fuse += seconds_since(t);
     }
     CachedFlush &cf = hit->second;
-    for (CachedKernel &ck : cf.kernels) {
-        ++stats.kernels;
-        for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
-        for (int o : ck.free_ordinals) release(ord2base[o]);
+    if (!cf.schedule_built) buildSchedule(cf);
+    if (cf.schedule && !dry_run) {
+        runSchedule(cf, ord2base, instr_list);
+    } else {
+        for (CachedKernel &ck : cf.kernels) {
+            ++stats.kernels;
+            for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
+            for (int o : ck.free_ordinals) release(ord2base[o]);
+        }
     }
     const bool index_checks = cf.index_checks;
     last_index_checks = index_checks;
@@ -542,6 +565,12 @@ void *EngineHip::getMemoryPointer(bh_base &base, bool copy2host, bool force_allo
     return ret;
 }
 
+void EngineHip::markDeviceWritten(bh_base *base) {
+    DeviceBuffer &d = buffer(base);
+    d.dev_valid = true;
+    d.host_valid = false;
+}
+
 void EngineHip::setMemoryPointer(bh_base *base, bool host_ptr, void *mem) {
     DeviceBuffer &d = buffer(base);
     if (host_ptr) {
@@ -583,6 +612,8 @@ std::string EngineHip::statistics() const {
     for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
     ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
     ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
+    ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
+    ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";
     ss << "pool_allocated: " << pool.allocated << "\npool_peak: " << pool.peak << "\npool_hits: " << pool.hits
        << "\npool_misses: " << pool.misses << "\n";
     return ss.str();
@@ -684,9 +715,15 @@ public:
             }
             engine.recording = nullptr;
             if (rec) {
-                if (i >= 2 && !cur.empty() && cur == prev && engine.quietToken() == q0) {
-                    graph = engine.graphFor(cur);
-                    graph_launches = cur.size();
+                if (i >= 2 && !cur.empty()) {
+                    if (cur != prev) {
+                        ++engine.graph_declined_changed;
+                    } else if (engine.quietToken() != q0) {
+                        ++engine.graph_declined_busy;
+                    } else {
+                        graph = engine.graphFor(cur);
+                        graph_launches = cur.size();
+                    }
                 }
                 prev.swap(cur);
             }
@@ -750,6 +787,9 @@ public:
             }
             ss << "max_grid=" << engine.codegenConfig().max_grid << " map_unroll=" << engine.codegenConfig().map_unroll
                << " reduce_unroll=" << engine.codegenConfig().reduce_unroll;
+        } else if (msg.rfind("batch_min:", 0) == 0) {
+            // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
+            engine.setBatchMin(std::stoi(msg.substr(10)));
         } else if (msg == "graph: on") {
             engine.graph_repeats = true;
         } else if (msg == "graph: off") {
@@ -794,7 +834,7 @@ public:
         void *s = engine.getMemoryPointer(*src.base, false, false, false);
         void *d = engine.getMemoryPointer(*dst.base, false, false, false);
         check(bhhip_memcpy_d2d(d, s, static_cast<size_t>(src.base->nbytes())), "memCopy");
-        engine.setMemoryPointer(dst.base, false, d);
+        engine.markDeviceWritten(dst.base);
     }
 
     void *getDeviceContext() override { return nullptr; }

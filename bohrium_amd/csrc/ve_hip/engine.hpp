@@ -12,6 +12,7 @@
 #pragma once
 
 #include <map>
+#include <memory>
 #include <set>
 #include <string>
 #include <unordered_map>
@@ -63,6 +64,8 @@ public:
     void handleExtmethod(BhIR *bhir);
     void *getMemoryPointer(bh_base &base, bool copy2host, bool force_alloc, bool nullify);
     void setMemoryPointer(bh_base *base, bool host_ptr, void *mem);
+    /* The device buffer of `base` now holds its newest data (written outside a kernel launch). */
+    void markDeviceWritten(bh_base *base);
     void copyAllToHost();
     void copyToHost(bh_base *base);
     std::string info() const;
@@ -101,6 +104,16 @@ public:
     void graphLaunch(void *graph, size_t nlaunches);
     bool graph_repeats = true;
     uint64_t graph_replays = 0;
+    /* Batched launches: groups of at least `batch_min` launches of one kernel code (0 disables). */
+    int batch_min = 8;
+    uint64_t batch_launches = 0, batched_items = 0;
+    void setBatchMin(int n) {
+        batch_min = n;
+        flush_cache.clear();
+    }
+    /* Repeat iterations not captured: launches differed from the previous iteration / the engine also
+     * allocated, copied or checked errors. */
+    uint64_t graph_declined_changed = 0, graph_declined_busy = 0;
     /* Kernel-writer settings; changing them through this reference drops every cached plan. */
     WriterConfig &codegenConfig() {
         flush_cache.clear();
@@ -133,11 +146,48 @@ private:
         std::vector<KernelPlan> plans;
         std::vector<int> free_ordinals;
     };
+    /* Many launches of one kernel code in a flush run as one batched launch (bh_kernel_batch):
+     * each item keeps its own packed arguments and workgroup count.  Launches are ordered in
+     * levels; a launch's level is one more than that of every earlier launch it conflicts with
+     * (reads what that one writes, or writes what it reads or writes). */
+    struct PlanRef {
+        uint32_t kernel, plan;
+    };
+    struct BatchGroup {
+        std::vector<PlanRef> items;
+        void *fn = nullptr, *fin_fn = nullptr;
+        unsigned char *dev = nullptr;  // packed arguments of every item, then first-workgroup table
+        uint64_t dev_bytes = 0;
+        uint8_t *host = nullptr;  // page-locked staging of the same
+        uint64_t host_bytes = 0;
+        void *uploaded = nullptr;  // event recorded after the last upload from `host`
+        unsigned char *scan_scratch = nullptr;  // zeroed look-back state, one region per item (SCAN)
+        uint64_t scan_bytes = 0;
+        ~BatchGroup();
+    };
+    struct Level {
+        std::vector<PlanRef> singles;
+        std::vector<size_t> groups;
+        std::vector<uint32_t> free_kernels;  // kernels whose frees are released after this level
+    };
+    struct Schedule {
+        std::vector<Level> levels;
+        std::vector<std::unique_ptr<BatchGroup> > groups;
+    };
     struct CachedFlush {
         std::vector<CachedKernel> kernels;
         bool index_checks = false;
         bool reusable = true;
+        bool schedule_built = false;
+        std::shared_ptr<Schedule> schedule;  // null: launch plan by plan
     };
+    void buildSchedule(CachedFlush &cf);
+    void runSchedule(CachedFlush &cf, const std::vector<bh_base *> &ord2base, const std::vector<bh_instruction *> &instrs);
+    void launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh_base *> &ord2base,
+                     const std::vector<bh_instruction *> &instrs);
+    size_t packArgs(const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
+                    const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap);
+    void compilePlan(KernelPlan &plan);
     std::unordered_map<std::string, CachedFlush> flush_cache;
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
     bool last_index_checks = false;

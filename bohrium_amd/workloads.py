@@ -104,6 +104,31 @@ def jacobi(n, iters):
     return G, deltas
 
 
+def batched_sizes(count=4096, lo=256, hi=262144, seed=42):
+    """Array sizes floor(exp(U[ln lo, ln hi])) from NumPy's seeded generator (host-side sizes)."""
+    rng = np.random.RandomState(seed)
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+
+
+def batched_inputs(sizes, seed=42):
+    """Per size n: x ~ N(0, 1) float32 of n elements and n int64 indices uniform in [0, n) (philox)."""
+    R = RandomState(seed)
+    xs = [R.standard_normal(int(n), dtype=np.float32) for n in sizes]
+    idxs = [R.randint(0, int(n), int(n)) for n in sizes]
+    return xs, idxs
+
+
+def batched_step(xs, idxs, a=0.5, b=0.1):
+    """Per array: y = tanh(a*x + b), sum(y), the int64 cumsum of (x > 0) and y[idx]."""
+    sums, cums, gathered = [], [], []
+    for x, idx in zip(xs, idxs):
+        y = ufunc("tanh")(x * a + b)
+        sums.append(y.sum())
+        cums.append((x > 0).astype(np.int64).cumsum())
+        gathered.append(_b.take(y, idx))
+    return sums, cums, gathered
+
+
 def jacobi_repeat(G, iters):
     """`iters` Jacobi sweeps of G in place as one recorded body replayed by the engine
     (do_while); returns the 0-d array with the last sweep's max |new - old|."""

@@ -43,6 +43,9 @@ int bhhip_free(void *dptr);
 int bhhip_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int bhhip_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int bhhip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+/* Host-to-device copy from page-locked memory (bhhip_host_alloc), ordered on the engine stream;
+ * returns at once.  The source must stay unchanged until an event recorded after it completes. */
+int bhhip_memcpy_h2d_async(void *dst, const void *src, size_t bytes);
 /* Zero device memory (used only outside graph capture, e.g. right after allocation). */
 int bhhip_memzero(void *dptr, size_t bytes);
 /* Page-locked host memory for staging copies. */
@@ -72,6 +75,8 @@ int bhhip_synchronize(void);
 int bhhip_event_create(void **event);
 int bhhip_event_record(void *event);
 int bhhip_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Wait until the work recorded before `event` has finished. */
+int bhhip_event_synchronize(void *event);
 int bhhip_event_destroy(void *event);
 
 /* Graph capture of the engine stream, instantiation and replay. */

@@ -4,6 +4,8 @@ Each test runs a program on the engine with the trace recorded, replays the
 identical trace on the oracle and compares: bit-exact for integer/bool/index/
 philox results; fp64 within 1e-12 and fp32 within 1e-5 relative (plus the
 stated absolute slack where a reduction order differs)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -313,7 +315,8 @@ def test_jacobi_repeat_graph_replay_matches_eager_and_oracle():
         bh.message("graph: on")
     before = int(bh.message("graph replays"))
     got, want = run_both(_jacobi_repeat_prog(258, 12))
-    assert int(bh.message("graph replays")) > before, "the repeat body was not replayed as a graph"
+    assert int(bh.message("graph replays")) > before, \
+        "the repeat body was not replayed as a graph:\n" + bh.message("statistic")
     for g, e in zip(got, eager):
         np.testing.assert_array_equal(g, e)
     for g, w in zip(got, want):
@@ -346,3 +349,47 @@ def test_unrolled_grid_stride_parity(unroll):
         assert int(got[4]) == int(want[4])
     finally:
         bh.message("codegen: " + old)
+
+
+def _batched_prog(count, seed=7):
+    sizes = workloads.batched_sizes(count, lo=256, hi=20000, seed=seed)
+
+    def prog():
+        xs, idxs = workloads.batched_inputs(sizes, seed=seed)
+        sums, cums, gathered = workloads.batched_step(xs, idxs)
+        return sums + cums + gathered + xs
+    return prog, count
+
+
+@pytest.mark.parametrize("count", [16, 200])
+def test_batched_uneven_blocks_parity(count):
+    """Many uneven arrays in one flush: the map+sum, scan+gather launches of every array run
+    as batched launches; results match the oracle (sums fp32, cumsums and gathers exact) and
+    the unbatched engine bit for bit."""
+    from bohrium_amd import _bh_api as api
+    prog, n = _batched_prog(count)
+    api.set_flush_threshold(10 ** 7)
+    try:
+        bh.message("batch_min: 0")
+        plain = [a.asnumpy() for a in prog()]
+        bh.message("batch_min: 8")
+        stats0 = bh.message("statistic")
+        before = int([l for l in stats0.splitlines() if l.startswith("batch_launches:")][0].split()[1])
+        got, want = run_both(prog)
+        stats1 = bh.message("statistic")
+        after = int([l for l in stats1.splitlines() if l.startswith("batch_launches:")][0].split()[1])
+    finally:
+        api.set_flush_threshold(1000)
+        bh.message("batch_min: 8")
+    assert after > before, "no batched launch:\n" + stats1
+    for g, p in zip(got, plain):
+        np.testing.assert_array_equal(g, p)
+    for i in range(n):
+        # fp32 sums in different orders: both within the bound n * 2^-24 * sum|y| of the exact sum
+        y = np.tanh(want[3 * n + i].astype(np.float64) * 0.5 + 0.1)
+        bound = y.size * 2.0 ** -24 * np.abs(y).sum()
+        exact = math.fsum(np.tanh(want[3 * n + i] * np.float32(0.5) + np.float32(0.1)).astype(np.float64))
+        assert abs(float(got[i]) - exact) <= bound + 1e-5 * abs(exact)
+        assert abs(float(want[i]) - exact) <= bound + 1e-5 * abs(exact)
+        np.testing.assert_array_equal(got[n + i], want[n + i])
+        assert_parity(got[2 * n + i], want[2 * n + i], np.float32, rtol=1e-5, atol=1e-6)
