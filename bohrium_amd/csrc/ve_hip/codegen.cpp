@@ -902,7 +902,10 @@ void Writer::emit_full_reduce() {
         total *= static_cast<uint64_t>(l1->size);
     }
     kp.work_items = total;
-    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((total + kp.block - 1) / kp.block, cfg.max_grid));
+    // Each thread folds at least reduce_min_items elements before another workgroup (and its
+    // partial) is added; at most reduce_max_grid workgroups.
+    const uint64_t per_wg = static_cast<uint64_t>(kp.block) * static_cast<uint64_t>(std::max(1, cfg.reduce_min_items));
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((total + per_wg - 1) / per_wg, cfg.reduce_max_grid));
     kp.grid = grid;
 
     // Kernel scope: every rank-0 reduction output is a per-thread accumulator that starts at the identity.
@@ -1061,29 +1064,31 @@ void Writer::emit_scan() {
     const int64_t tile_elems = static_cast<int64_t>(kp.block) * items;
     const int64_t ntiles = (l0.size + tile_elems - 1) / tile_elems;
     kp.grid = static_cast<uint64_t>(std::max<int64_t>(1, ntiles));
-    // Scratch layout: [epoch u32 | tile counter u32 | pad to 256], then one 48-byte record per tile
-    // (flag u32 at +0, aggregate at +16, inclusive prefix at +32); no offset depends on the size.
-    kp.scratch_bytes = 256 + ntiles * 48;
+    // Scratch layout: [tile counter u64 = epoch << 32 | tiles taken, pad to 256], then one record
+    // of 8 bytes per 32-bit chunk of T per tile; no offset depends on the size.
+    const int64_t nchunks = std::max<int64_t>(1, bh_type_size(t) / 4);
+    kp.scratch_bytes = 256 + ntiles * 8 * nchunks;
     const std::string ident = literal(jitk::sweep_identity(acc->opcode, t));
     scan_instr = acc.get();
     scan_sink = "x_";
 
     body << "  auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
          << combine_expr(acc->opcode, "x", "y") << "; };\n";
-    body << "  unsigned int *bh_epoch_p = (unsigned int *)bh_scratch;\n";
-    body << "  unsigned int *bh_tiles_p = bh_epoch_p + 1;\n";
     body << "  unsigned char *bh_rec = bh_scratch + 256;\n";
-    body << "#define BH_FLAG(j) ((unsigned int *)(bh_rec + (size_t)(j) * 48))\n";
-    body << "#define BH_AGG(j) (*(" << T << " *)(bh_rec + (size_t)(j) * 48 + 16))\n";
-    body << "#define BH_INC(j) (*(" << T << " *)(bh_rec + (size_t)(j) * 48 + 32))\n";
     body << "  __shared__ unsigned int bh_tile_s, bh_ep_s;\n";
     body << "  __shared__ " << T << " bh_wtot_[BH_WAVES];\n";
     body << "  __shared__ " << T << " bh_excl_s;\n";
+    // Tiles are numbered in the order workgroups start, so a tile only waits for tiles whose
+    // workgroups already run.  The workgroup taking the last tile starts the next epoch.
     body << "  if (threadIdx.x == 0) {\n";
-    body << "    bh_ep_s = __hip_atomic_load(bh_epoch_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
-    body << "    bh_tile_s = atomicAdd(bh_tiles_p, 1u);\n  }\n";
+    body << "    const unsigned long long t_ = atomicAdd((unsigned long long *)bh_scratch, 1ull);\n";
+    body << "    bh_tile_s = (unsigned int)t_;\n    bh_ep_s = (unsigned int)(t_ >> 32);\n";
+    body << "    if ((unsigned int)t_ == bh_nblk - 1u)\n";
+    body << "      __hip_atomic_store((unsigned long long *)bh_scratch, (unsigned long long)(bh_ep_s + 1u) << 32, "
+            "__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+    body << "  }\n";
     body << "  __syncthreads();\n";
-    body << "  const int64_t tile_ = bh_tile_s;\n  const unsigned int ep_ = bh_ep_s * 4u;\n";
+    body << "  const int64_t tile_ = bh_tile_s;\n  const unsigned int fb_ = bh_ep_s << 2;\n";
     body << "  const int lane_ = threadIdx.x & 63, wave_ = threadIdx.x >> 6;\n";
     body << "  const int64_t tb_ = tile_ * " << tile_elems << ";\n";
     body << "  " << T << " v_[" << items << "];\n";
@@ -1108,25 +1113,29 @@ void Writer::emit_scan() {
     body << "  if (lane_ == 63) bh_wtot_[wave_] = carry_;\n  __syncthreads();\n";
     body << "  " << T << " wpre_ = " << ident << ", ttot_ = " << ident << ";\n";
     body << "  for (int w_ = 0; w_ < BH_WAVES; ++w_) { if (w_ < wave_) wpre_ = op_(wpre_, bh_wtot_[w_]); ttot_ = op_(ttot_, bh_wtot_[w_]); }\n";
-    body << "  if (threadIdx.x == 0) {\n";
+    // Look-back by wave 0: lane l reads the record of tile (hi - l); the window ends at the
+    // nearest inclusive prefix, and the aggregates after it are folded in.
+    const std::string LB = "bh_lookback<" + T + ">";
+    body << "  if (wave_ == 0) {\n";
     body << "    " << T << " ex_ = " << ident << ";\n";
-    body << "    if (tile_ > 0) {\n";
-    body << "      BH_AGG(tile_) = ttot_;\n";
-    body << "      __hip_atomic_store(BH_FLAG(tile_), ep_ + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
-    body << "      for (int64_t j_ = tile_ - 1;; --j_) {\n";
-    body << "        unsigned int f_;\n";
-    body << "        do { f_ = __hip_atomic_load(BH_FLAG(j_), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }"
-            " while (f_ != ep_ + 1u && f_ != ep_ + 2u);\n";
-    body << "        if (f_ == ep_ + 2u) { ex_ = op_(BH_INC(j_), ex_); break; }\n";
-    body << "        ex_ = op_(BH_AGG(j_), ex_);\n";
-    body << "      }\n    }\n";
-    body << "    BH_INC(tile_) = op_(ex_, ttot_);\n";
-    body << "    __hip_atomic_store(BH_FLAG(tile_), ep_ + 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);\n";
-    body << "    bh_excl_s = ex_;\n";
-    body << "    if (tile_ == (int64_t)bh_nblk - 1) {\n";
-    body << "      *bh_tiles_p = 0u;\n";
-    body << "      __hip_atomic_store(bh_epoch_p, bh_ep_s + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
-    body << "    }\n  }\n";
+    body << "    if (tile_ == 0) {\n";
+    body << "      if (lane_ == 0) " << LB << "::put(bh_rec, 0, ttot_, fb_ | 2u);\n";
+    body << "    } else {\n";
+    body << "      if (lane_ == 0) " << LB << "::put(bh_rec, tile_, ttot_, fb_ | 1u);\n";
+    body << "      for (int64_t hi_ = tile_ - 1;; hi_ -= 64) {\n";
+    body << "        const int64_t j_ = hi_ - lane_;\n";
+    body << "        " << T << " w_ = " << ident << ";\n";
+    body << "        unsigned int st_ = 2u;\n";
+    body << "        if (j_ >= 0) { while ((st_ = " << LB << "::get(bh_rec, j_, fb_, &w_)) == 0u) {} }\n";
+    body << "        const unsigned long long pm_ = __ballot(st_ == 2u);\n";
+    body << "        const int first_ = pm_ ? __ffsll((long long)pm_) - 1 : 64;\n";
+    body << "        ex_ = op_(bh_wave_reduce(lane_ <= first_ ? w_ : " << ident << ", op_), ex_);\n";
+    body << "        if (pm_) break;\n";
+    body << "      }\n";
+    body << "      if (lane_ == 0) " << LB << "::put(bh_rec, tile_, op_(ex_, ttot_), fb_ | 2u);\n";
+    body << "    }\n";
+    body << "    if (lane_ == 0) bh_excl_s = ex_;\n";
+    body << "  }\n";
     body << "  __syncthreads();\n";
     body << "  const " << T << " pre_ = op_(bh_excl_s, wpre_);\n";
     const bh_view &out = acc->operand[0];
@@ -1134,7 +1143,6 @@ void Writer::emit_scan() {
     body << "    const int64_t i0 = tb_ + ((int64_t)wave_ * " << items << " + j_) * 64 + lane_;\n";
     body << "    if (i0 < n" << lid.at(&l0) << ") a" << bid.at(out.base) << "[" << index(out) << "] = op_(pre_, v_[j_]);\n";
     body << "  }\n";
-    body << "#undef BH_FLAG\n#undef BH_AGG\n#undef BH_INC\n";
     write_back(unit, ksc, 1);
 }
 

@@ -82,7 +82,9 @@ struct KernelPlan {
 
 /* Hooks the engine supplies to the writer. */
 struct WriterConfig {
-    int max_grid = 2048;     // workgroups for grid-stride skeletons
+    int max_grid = 2048;     // workgroups for the MAP skeleton
+    int reduce_max_grid = 4096;  // workgroups for FULL_REDUCE
+    int reduce_min_items = 4;    // FULL_REDUCE: elements per thread before another workgroup is added
     int scan_items = 8;      // elements per thread per scan tile
     int map_unroll = 1;      // grid-stride elements per thread issued together (MAP)
     int reduce_unroll = 1;   // the same for FULL_REDUCE

@@ -115,6 +115,8 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     fusion_config.merge_filter = [](const jitk::LoopB &a, const jitk::LoopB &b) { return root_merge_allowed(a, b); };
     fusion_config.accumulate_alone = true;
     wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
+    wcfg.reduce_max_grid = cfg.defaultGet<int>("reduce_max_grid", 4096);
+    wcfg.reduce_min_items = std::max(1, cfg.defaultGet<int>("reduce_min_items", 4));
     wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
     wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
     wcfg.reduce_unroll = std::max(1, cfg.defaultGet<int>("reduce_unroll", 1));
@@ -781,12 +783,17 @@ public:
                     engine.codegenConfig().map_unroll = v;
                 } else if (k == "reduce_unroll") {
                     engine.codegenConfig().reduce_unroll = v;
+                } else if (k == "reduce_max_grid") {
+                    engine.codegenConfig().reduce_max_grid = v;
+                } else if (k == "reduce_min_items") {
+                    engine.codegenConfig().reduce_min_items = v;
                 } else {
                     throw std::invalid_argument("hip engine: unknown codegen setting: " + k);
                 }
             }
-            ss << "max_grid=" << engine.codegenConfig().max_grid << " map_unroll=" << engine.codegenConfig().map_unroll
-               << " reduce_unroll=" << engine.codegenConfig().reduce_unroll;
+            const WriterConfig &w = engine.codegenConfig();
+            ss << "max_grid=" << w.max_grid << " map_unroll=" << w.map_unroll << " reduce_unroll=" << w.reduce_unroll
+               << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items;
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
