@@ -3,15 +3,17 @@
 Workload (one step = one full run of the configuration): S ~ U[58,62], K ~ U[55,65],
 T ~ U[0.25,2] float64, N = 2e7 (generated on the device by philox before timing);
 10 time steps, each a fused map (A&S polynomial CND with exp/log/sqrt) + sum.  The
-time steps are one do_while body replayed by the engine; all 10 sums are fetched
-once at the end of each run.
+time steps are one do_while body replayed by the engine.  Steps are issued back to
+back (the engine's stream is asynchronous); the timed region ends with one
+synchronizing fetch of every step's 10 sums.
 Algorithmic bytes per time step = read S, K, T once = 24 * N bytes.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-secondary]
 
 Prints one JSON line with the value (GB/s over the whole timed job), the roofline
 of the dominant kernel (per-launch device time from HIP events on the engine
-stream) and the CPU baseline (the oracle's fused OpenMP form on the host cores).
+stream), the CPU baseline (the oracle's fused OpenMP form on the host cores) and
+secondary measurements of the 2-D Jacobi and batched-arrays workloads.
 """
 import argparse
 import json
@@ -29,17 +31,20 @@ TIME_STEPS = 10
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--n", type=int, default=20_000_000)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, default=20_000_000)
+    p.add_argument("--no-secondary", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a plain 256 MiB device copy 5 times (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
 
 
 def traffic_from_profiles():
     """HBM bytes per launch of the Black-Scholes kernel measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), if recorded."""
     path = os.path.join(ROOT, "profiles", "black_scholes_step_traffic.json")
     if not os.path.exists(path):
         return None
@@ -48,7 +53,6 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(n):
-    import numpy as np
     import bohrium_amd as bh
     from bohrium_amd import workloads, _bh_api as api
     from oracle import omp_ve
@@ -68,6 +72,71 @@ def cpu_baseline(n):
                       "-fopenmp), %d threads" % (n, TIME_STEPS, threads)}
 
 
+def launches(bh):
+    return [l.split() for l in bh.message("launches").splitlines() if l.strip()]
+
+
+def jacobi_secondary(bh, workloads):
+    """2-D Jacobi fp32 8192^2, 200 sweeps replayed as one graph per sweep."""
+    n, iters = 8192, 200
+    G = workloads.jacobi_grid(n)
+    workloads.jacobi_repeat(G, 5).asnumpy()
+    bh.message("profile: on")
+    launches(bh)
+    workloads.jacobi_repeat(G, 3).asnumpy()
+    ls = launches(bh)
+    bh.message("profile: off")
+    red = [float(l[3]) for l in ls if l[0] == "reduce"]
+    mp = [float(l[3]) for l in ls if l[0] == "map" and int(l[1]) > 1]
+    before = int(bh.message("graph replays"))
+    t0 = time.perf_counter()
+    d = workloads.jacobi_repeat(G, iters).asnumpy()
+    dt = time.perf_counter() - t0
+    replays = int(bh.message("graph replays")) - before
+    alg = n * n * 4 + (n - 2) * (n - 2) * 4
+    return {"workload": "jacobi fp32 %dx%d, %d sweeps (stencil + max|delta| + copy-back), do_while replayed as a HIP graph"
+                        % (n, n, iters),
+            "ms_per_sweep": round(dt * 1e3 / iters, 4), "effective_GBs": round(alg * iters / dt / 1e9, 1),
+            "algorithmic_bytes_per_sweep": alg, "graph_replays": replays,
+            "kernel_ms": {"stencil+max (reduce)": round(sum(red) / max(1, len(red)), 4),
+                          "copy-back (map)": round(sum(mp) / max(1, len(mp)), 4)},
+            "last_delta": float(d)}
+
+
+def batched_secondary(bh, workloads, api):
+    """4096 uneven float32 arrays (256..262144 elements) in one flush: batched vs one launch per block."""
+    sizes = workloads.batched_sizes(4096)
+    elems = int(sizes.sum())
+    api.set_flush_threshold(10 ** 7)
+    xs, idxs = workloads.batched_inputs(sizes)
+    bh.flush()
+    out = {"workload": "4096 float32 arrays, sizes floor(exp(U[ln 256, ln 262144])) (seed 42, %d elements): "
+                       "tanh(0.5x+0.1), sum, int64 cumsum(x>0), y[idx]" % elems,
+           "algorithmic_bytes": 36 * elems}
+
+    def run():
+        sums, cums, gathered = workloads.batched_step(xs, idxs)
+        bh.asnumpy_many([sums[-1], cums[-1][:1], gathered[-1][:1]])
+
+    for label, bm in (("batched", 8), ("one_launch_per_block", 0)):
+        bh.message("batch_min: %d" % bm)
+        run()
+        t0 = time.perf_counter()
+        run()
+        dt = time.perf_counter() - t0
+        bh.message("profile: on")
+        launches(bh)
+        run()
+        ls = launches(bh)
+        bh.message("profile: off")
+        dev = sum(float(l[3]) for l in ls)
+        out[label] = {"launches": len(ls), "device_ms": round(dev, 3), "wall_ms": round(dt * 1e3, 1),
+                      "device_GBs": round(36.0 * elems / (dev * 1e-3) / 1e9, 1) if dev > 0 else None}
+    bh.message("batch_min: 8")
+    api.set_flush_threshold(1000)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -75,7 +144,7 @@ def main():
               "sharding design)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import bohrium_amd as bh
-    from bohrium_amd import workloads
+    from bohrium_amd import workloads, _bh_api as api
 
     n = args.n
     S, K, T = workloads.black_scholes_inputs(n)
@@ -83,32 +152,48 @@ def main():
 
     def run():
         # The 10 time steps are one recorded loop body replayed by the engine (do_while with a
-        # sliding view for the time offset and the output slot); one sync at the end.
-        return workloads.black_scholes_repeat(S, K, T, TIME_STEPS).asnumpy()
+        # sliding view for the time offset and the output slot); nothing is fetched here.
+        return workloads.black_scholes_repeat(S, K, T, TIME_STEPS)
 
     for _ in range(args.warmup):
-        run()
+        run().asnumpy()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        vals = run()
+    outs = [run() for _ in range(args.steps)]
+    vals = bh.asnumpy_many(outs)
     elapsed = time.perf_counter() - t0
     bytes_per_run = 24.0 * n * TIME_STEPS
     value = bytes_per_run * args.steps / elapsed / 1e9
+    del outs
 
-    # Per-launch device time of the dominant kernel (HIP events on the engine stream).
+    # Per-launch device time of the dominant kernel (HIP events on the engine stream): the
+    # main kernel and its one-workgroup finalize kernel together.
     bh.message("profile: on")
-    bh.message("launches")
-    for _ in range(max(2, args.steps)):
-        run()
-    lines = [l.split() for l in bh.message("launches").splitlines() if l.strip()]
+    launches(bh)
+    for _ in range(max(2, args.steps // 2)):
+        run().asnumpy()
+    ls = launches(bh)
     bh.message("profile: off")
-    main_k = [float(l[3]) for l in lines if l[0] == "reduce" and int(l[1]) == n]
+    main_k = [float(l[3]) for l in ls if l[0] == "reduce" and int(l[1]) == n]
     avg_ms = sum(main_k) / len(main_k) if main_k else float("nan")
-    achieved = 24.0 * n / (avg_ms * 1e-3) / 1e9
+    achieved = 24.0 * n / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else float("nan")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                "kernel": "black-scholes fused map+reduce (skeleton: reduce)", "avg_kernel_ms": round(avg_ms, 4),
-                "launches_measured": len(main_k)}
+                "kernel": "black-scholes step: fused map+sum (FULL_REDUCE main + finalize)",
+                "avg_kernel_ms": round(avg_ms, 4), "launches_measured": len(main_k),
+                "algorithmic_bytes_per_launch": int(24 * n)}
+
+    if args.calibrate_copy:
+        a = bh.random.RandomState(1).random_sample(32 * 1024 * 1024)
+        b = bh.empty(32 * 1024 * 1024)
+        for _ in range(5):
+            b[...] = a
+            bh.asnumpy_many([b[:1]])
+        del a, b
+
+    secondary = None
+    if not args.no_secondary:
+        del S, K, T
+        secondary = {"jacobi": jacobi_secondary(bh, workloads), "batched": batched_secondary(bh, workloads, api)}
 
     cpu = None
     if not args.no_cpu:
@@ -124,7 +209,8 @@ def main():
            "roofline": roofline, "cpu_baseline": cpu,
            "seconds_per_run": round(elapsed / args.steps, 5),
            "speedup_vs_cpu_baseline": (round(cpu["seconds_per_run"] / (elapsed / args.steps), 1) if cpu else None),
-           "check": {"first_step_sum": float(vals[0]), "last_step_sum": float(vals[-1])}}
+           "secondary": secondary,
+           "check": {"first_step_sum": float(vals[0][0]), "last_step_sum": float(vals[-1][-1])}}
     print(json.dumps(out))
 
 

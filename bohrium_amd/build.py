@@ -30,8 +30,8 @@ CORE = ["core/bh_type.cpp", "core/bh_constant.cpp", "core/bh_view.cpp", "core/bh
         "core/bh_main_memory.cpp", "jitk/block.cpp", "jitk/fuser.cpp"]
 BRIDGE = ["bridge/runtime.cpp"]
 NODE = ["vem_node/node.cpp"]
-VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp"]
-VE_HIP_HIP = ["ve_hip/bhhip.cpp", "ve_hip/engine.cpp", "ve_hip/batch.cpp"]
+VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp", "ve_hip/features.cpp"]
+VE_HIP_HIP = ["ve_hip/bhhip.cpp", "ve_hip/engine.cpp", "ve_hip/batch.cpp", "ve_hip/costmodel.hip"]
 
 
 def _run(cmd):

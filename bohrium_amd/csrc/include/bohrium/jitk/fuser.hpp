@@ -29,6 +29,9 @@ struct FusionConfig {
     bool avoid_rank0_sweep = false;
     /* Used by the "learned" transformer; when empty it falls back to the analytic weight. */
     MergeWeight learned_weight;
+    /* Used by the "learned" fuser when set: the predicted device time of a block made of each
+     * given instruction set (false: no model, fall back to the analytic greedy fuser). */
+    std::function<bool(const std::vector<std::vector<InstrPtr> > &, std::vector<double> &)> learned_cost;
     /* Engine-specific veto on merging two blocks (e.g. keep root blocks on a parallel skeleton). */
     std::function<bool(const LoopB &, const LoopB &)> merge_filter;
     /* Keep an accumulate out of the pre-fuser's groups that hold any other sweep (a GPU scan
@@ -46,6 +49,10 @@ std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_li
 std::vector<Block> fuser_singleton(const std::vector<bh_instruction *> &instr_list);
 void fuser_serial(std::vector<Block> &block_list, const FusionConfig &config);
 void fuser_greedy(const FusionConfig &config, std::vector<Block> &block_list, const MergeWeight &weight_fn);
+/* Greedy fusion scored by a cost model: repeatedly merge the pair whose merge is predicted to
+ * save the most time, cost(a) + cost(b) - cost(a u b), while that saving is positive.  Returns
+ * false (block_list untouched) when config.learned_cost has no model. */
+bool fuser_greedy_learned(const FusionConfig &config, std::vector<Block> &block_list);
 void push_reductions_inwards(std::vector<Block> &block_list);
 void collapse_redundant_axes(std::vector<Block> &block_list);
 

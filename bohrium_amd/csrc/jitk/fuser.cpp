@@ -424,6 +424,71 @@ void fuser_greedy(const FusionConfig &config, std::vector<Block> &block_list, co
         if (!b.isInstr()) fuser_greedy(config, b.getLoop()._block_list, weight_fn);
 }
 
+bool fuser_greedy_learned(const FusionConfig &config, std::vector<Block> &block_list) {
+    if (!config.learned_cost) return false;
+    Dag dag = from_block_list(block_list);
+    if (dag.num_edges() > config.greedy_threshold) return false;
+    std::vector<double> node_cost(dag.node.size(), -1.0);
+    bool first = true;
+    while (true) {
+        std::vector<std::pair<int, int> > fusibles;
+        for (size_t a = 0; a < dag.node.size(); ++a) {
+            if (!dag.alive[a]) continue;
+            for (int b : std::vector<int>(dag.child[a].begin(), dag.child[a].end())) {
+                if (dag.long_path(static_cast<int>(a), b)) {
+                    dag.remove_edge(static_cast<int>(a), b);
+                } else if (config.can_merge(dag.node[a], dag.node[b])) {
+                    fusibles.emplace_back(static_cast<int>(a), b);
+                }
+            }
+        }
+        if (fusibles.empty()) break;
+        // One cost query per round: every node without a cached cost, then every candidate union.
+        std::vector<std::vector<InstrPtr> > sets;
+        std::vector<int> node_of_set;
+        std::set<int> need;
+        for (const auto &e : fusibles) {
+            if (node_cost[e.first] < 0) need.insert(e.first);
+            if (node_cost[e.second] < 0) need.insert(e.second);
+        }
+        for (int v : need) {
+            sets.push_back(allInstr(dag.node[v]));
+            node_of_set.push_back(v);
+        }
+        for (const auto &e : fusibles) {
+            std::vector<InstrPtr> u = allInstr(dag.node[e.first]);
+            const std::vector<InstrPtr> b = allInstr(dag.node[e.second]);
+            u.insert(u.end(), b.begin(), b.end());
+            sets.push_back(std::move(u));
+        }
+        std::vector<double> cost;
+        if (!config.learned_cost(sets, cost) || cost.size() != sets.size()) {
+            if (first) return false;
+            throw std::runtime_error("learned fuser: the cost model failed");
+        }
+        first = false;
+        for (size_t k = 0; k < node_of_set.size(); ++k) node_cost[node_of_set[k]] = cost[k];
+        double best_w = 0.0;
+        int best = -1;
+        for (size_t k = 0; k < fusibles.size(); ++k) {
+            const auto &e = fusibles[k];
+            const double w = node_cost[e.first] + node_cost[e.second] - cost[node_of_set.size() + k];
+            if (w > best_w) {
+                best_w = w;
+                best = static_cast<int>(k);
+            }
+        }
+        if (best < 0) break;  // no merge is predicted to save time
+        dag.merge(fusibles[best].first, fusibles[best].second);
+        node_cost[fusibles[best].first] = -1.0;
+    }
+    block_list = dag.topo();
+    const MergeWeight analytic = [](const Block &a, const Block &b) { return static_cast<double>(weight(a, b)); };
+    for (Block &b : block_list)
+        if (!b.isInstr()) fuser_greedy(config, b.getLoop()._block_list, analytic);
+    return true;
+}
+
 // ---------------------------------------------------------------- transformers
 
 namespace {
@@ -637,7 +702,8 @@ std::vector<Block> get_block_list(const std::vector<bh_instruction *> &instr_lis
         if (t == "greedy") {
             fuser_greedy(config, block_list, analytic);
         } else if (t == "learned") {
-            fuser_greedy(config, block_list, config.learned_weight ? config.learned_weight : analytic);
+            if (!fuser_greedy_learned(config, block_list))
+                fuser_greedy(config, block_list, config.learned_weight ? config.learned_weight : analytic);
         } else if (t == "serial") {
             fuser_serial(block_list, config);
         } else if (t == "breadth_first") {

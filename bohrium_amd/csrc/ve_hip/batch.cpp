@@ -126,7 +126,10 @@ void EngineHip::runSchedule(CachedFlush &cf, const std::vector<bh_base *> &ord2b
     stats.kernels += cf.kernels.size();
     Schedule &s = *cf.schedule;
     for (Level &lv : s.levels) {
-        for (const PlanRef &r : lv.singles) launchPlan(cf.kernels[r.kernel].plans[r.plan], ord2base, instrs);
+        for (const PlanRef &r : lv.singles) {
+            kernel_seq = r.kernel + 1000000000ull;
+            launchPlan(cf.kernels[r.kernel].plans[r.plan], ord2base, instrs);
+        }
         for (size_t g : lv.groups) launchBatch(*s.groups[g], cf, ord2base, instrs);
         for (uint32_t k : lv.free_kernels)
             for (int o : cf.kernels[k].free_ordinals) release(ord2base[o]);
@@ -257,7 +260,7 @@ void EngineHip::launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh
     if (profile) {
         bhhip_event_record(ev1);
         pending_events.push_back({ev0, ev1, {std::string("batch-") + strategy_name(p0.strategy), work, blocks, 0.0,
-                                             "items=" + std::to_string(k)}});
+                                             "items=" + std::to_string(k), 0, ""}});
     }
     stats.time_launch += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
 }

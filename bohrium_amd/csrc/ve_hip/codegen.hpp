@@ -75,6 +75,8 @@ struct KernelPlan {
     std::vector<int> param_ordinals, written_ordinals;
     void *function = nullptr;
     void *fin_function = nullptr;
+    /* Cost-model features of the jitk kernel this launch belongs to (recorded with profiled launches). */
+    std::vector<float> features;
     uint64_t source_hash = 0;
     /* False when a sliding view's start cannot be re-read from the flush (plan not reusable). */
     bool reusable = true;

@@ -114,6 +114,9 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     fusion_config.avoid_rank0_sweep = false;
     fusion_config.merge_filter = [](const jitk::LoopB &a, const jitk::LoopB &b) { return root_merge_allowed(a, b); };
     fusion_config.accumulate_alone = true;
+    fusion_config.learned_cost = [](const std::vector<std::vector<jitk::InstrPtr> > &sets, std::vector<double> &us) {
+        return predict_block_times(sets, us);
+    };
     wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
     wcfg.reduce_max_grid = cfg.defaultGet<int>("reduce_max_grid", 4096);
     wcfg.reduce_min_items = std::max(1, cfg.defaultGet<int>("reduce_min_items", 4));
@@ -270,7 +273,8 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
         bhhip_event_record(ev1);
         char hex[32];
         std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(plan.source_hash));
-        pending_events.push_back({ev0, ev1, {strategy_name(plan.strategy), plan.work_items, plan.grid, 0.0, hex}});
+        pending_events.push_back({ev0, ev1, {strategy_name(plan.strategy), plan.work_items, plan.grid, 0.0, hex,
+                                             kernel_seq, features_string(plan.features)}});
     }
     stats.time_launch += seconds_since(t);
 }
@@ -461,6 +465,8 @@ void EngineHip::handleExecution(BhIR *bhir) {
             CachedKernel ck;
             if (!kernel.isSystemOnly()) {
                 ck.plans = plan_kernel(kernel, wcfg);
+                const std::vector<float> feats = block_features(jitk::allInstr(kernel));
+                for (KernelPlan &p : ck.plans) p.features = feats;
                 for (KernelPlan &p : ck.plans) {
                     for (const ArgSpec &a : p.args)
                         if (a.view_origin >= static_cast<int64_t>(instr_list.size())) p.reusable = false;
@@ -495,6 +501,7 @@ void EngineHip::handleExecution(BhIR *bhir) {
     } else {
         for (CachedKernel &ck : cf.kernels) {
             ++stats.kernels;
+            ++kernel_seq;
             for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
             for (int o : ck.free_ordinals) release(ord2base[o]);
         }
@@ -787,16 +794,21 @@ public:
                     engine.codegenConfig().reduce_max_grid = v;
                 } else if (k == "reduce_min_items") {
                     engine.codegenConfig().reduce_min_items = v;
+                } else if (k == "scan_items") {
+                    engine.codegenConfig().scan_items = v;
                 } else {
                     throw std::invalid_argument("hip engine: unknown codegen setting: " + k);
                 }
             }
             const WriterConfig &w = engine.codegenConfig();
             ss << "max_grid=" << w.max_grid << " map_unroll=" << w.map_unroll << " reduce_unroll=" << w.reduce_unroll
-               << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items;
+               << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items
+               << " scan_items=" << w.scan_items;
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
+        } else if (msg == "plans: drop") {
+            engine.dropPlans();
         } else if (msg == "graph: on") {
             engine.graph_repeats = true;
         } else if (msg == "graph: off") {
@@ -810,7 +822,7 @@ public:
             engine.resolveProfile();
             for (const auto &r : engine.last_launches)
                 ss << r.strategy << " " << r.work_items << " " << r.grid << " " << r.device_ms << " " << r.kernel_hash
-                   << "\n";
+                   << " k=" << r.kernel_id << " f=" << r.features << "\n";
             engine.last_launches.clear();
         } else if (msg.rfind("fuser_list:", 0) == 0) {
             std::vector<std::string> list;
@@ -822,6 +834,7 @@ public:
                 if (!item.empty()) list.push_back(item);
             }
             engine.fusion_config.fuser_list = list;
+            engine.dropPlans();
         }
         if (child.initiated()) ss << child.message(msg);
         return ss.str();

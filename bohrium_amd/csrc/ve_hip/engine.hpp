@@ -21,6 +21,7 @@
 #include <bohrium/jitk/fuser.hpp>
 #include "codegen.hpp"
 #include "rewrite.hpp"
+#include "features.hpp"
 
 namespace bohrium {
 namespace vehip {
@@ -79,6 +80,8 @@ public:
         uint64_t grid;
         double device_ms;
         std::string kernel_hash;
+        uint64_t kernel_id = 0;   // executed jitk kernel this launch belongs to
+        std::string features;     // its cost-model features (comma separated)
     };
     std::vector<LaunchRecord> last_launches;
     /* Resolve the events of profiled launches into last_launches (synchronizes). */
@@ -104,6 +107,7 @@ public:
     void graphLaunch(void *graph, size_t nlaunches);
     bool graph_repeats = true;
     uint64_t graph_replays = 0;
+    uint64_t kernel_seq = 0;  // id of the jitk kernel whose plans are being launched
     /* Batched launches: groups of at least `batch_min` launches of one kernel code (0 disables). */
     int batch_min = 8;
     uint64_t batch_launches = 0, batched_items = 0;
@@ -111,6 +115,8 @@ public:
         batch_min = n;
         flush_cache.clear();
     }
+    /* Forget every cached flush plan (after a change of fusion settings or of the cost model). */
+    void dropPlans() { flush_cache.clear(); }
     /* Repeat iterations not captured: launches differed from the previous iteration / the engine also
      * allocated, copied or checked errors. */
     uint64_t graph_declined_changed = 0, graph_declined_busy = 0;

@@ -129,6 +129,69 @@ def batched_step(xs, idxs, a=0.5, b=0.1):
     return sums, cums, gathered
 
 
+def random_dag(seed, n_instr=100, sides=(512, 1024, 2048), live=4):
+    """A seeded random array program of about `n_instr` bytecode instructions on float32
+    arrays: element-wise add/multiply/exp/sqrt, axis reductions (broadcast back), transposed,
+    reversed and half-size slice views.  At most `live` arrays per shape stay referenced
+    (older ones are freed, so the fuser can turn them into temporaries).  Returns the
+    arrays still live at the end; fetching them runs the whole program."""
+    rng = np.random.RandomState(seed)
+    s = int(rng.choice(sides))
+    R = RandomState(seed)
+    pools = {}
+
+    def add(a):
+        p = pools.setdefault(a.shape, [])
+        p.append(a)
+        if len(p) > live:
+            p.pop(0)
+
+    def pick(shape=None):
+        shapes = [k for k, v in pools.items() if v and (shape is None or k == shape)]
+        k = shapes[rng.randint(len(shapes))]
+        return pools[k][rng.randint(len(pools[k]))]
+
+    for _ in range(3):
+        add(R.random_sample((s, s)).astype(np.float32) * 2.0 - 1.0)
+    count = 0
+    sqrt, exp, absolute = ufunc("sqrt"), ufunc("exp"), ufunc("absolute")
+    while count < n_instr:
+        op = rng.choice(["add", "mul", "exp", "sqrt", "reduce", "transpose", "reverse", "half"],
+                        p=[0.25, 0.2, 0.1, 0.1, 0.1, 0.1, 0.1, 0.05])
+        a = pick()
+        if op in ("add", "mul"):
+            b = pick(a.shape)
+            if rng.rand() < 0.3 and a.ndim == 2 and a.shape[0] == a.shape[1]:
+                b = b.T
+            c = (a + b) * 0.5 if op == "add" else a * b
+            count += 2 if op == "add" else 1
+        elif op == "exp":
+            c = exp(a) * 0.3
+            count += 2
+        elif op == "sqrt":
+            c = sqrt(absolute(a))
+            count += 2
+        elif op == "reduce" and a.ndim == 2:
+            axis = int(rng.randint(2))
+            m = a.sum(axis=axis) * (1.0 / a.shape[axis])
+            c = a - (m[None, :] if axis == 0 else m[:, None])
+            count += 3
+        elif op == "transpose" and a.ndim == 2 and a.shape[0] == a.shape[1]:
+            c = a.T * 1.0
+            count += 1
+        elif op == "reverse" and a.ndim == 2:
+            c = a[::-1, :] - a[:, ::-1]
+            count += 1
+        elif op == "half" and a.ndim == 2 and a.shape[0] >= 4:
+            h = a.shape[0] // 2
+            c = a[:h] + a[h:2 * h]
+            count += 1
+        else:
+            continue
+        add(c)
+    return [a for p in pools.values() for a in p]
+
+
 def jacobi_repeat(G, iters):
     """`iters` Jacobi sweeps of G in place as one recorded body replayed by the engine
     (do_while); returns the 0-d array with the last sweep's max |new - old|."""

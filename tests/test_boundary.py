@@ -15,6 +15,7 @@ HEADER_LIBS = {
     "bhhip.h": "libbh_ve_hip.so",
     "bhc.h": "libbhc.so",
     "bh_ve_hip.h": "libbh_ve_hip.so",
+    "bh_costmodel.h": "libbh_ve_hip.so",
 }
 
 

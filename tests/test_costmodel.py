@@ -1,0 +1,105 @@
+"""The learned cost model: the float64 oracle network pinned by finite differences (CPU),
+and the HIP training/inference kernels checked against it (MI355X)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+F, H = 16, 8
+
+
+def _data(n=40, seed=0):
+    rng = np.random.RandomState(seed)
+    x = rng.normal(size=(n, F))
+    t = np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2] - 0.2 * x[:, 3]
+    p = rng.normal(scale=0.4, size=mlp.layout(F, H)["n"])
+    mean = rng.normal(scale=0.1, size=F)
+    inv_std = rng.uniform(0.5, 1.5, size=F)
+    return x, t, p, mean, inv_std
+
+
+def test_oracle_gradient_matches_finite_differences():
+    x, t, p, mean, inv_std = _data()
+    loss, g = mlp.loss_and_gradient(p, x, t, F, H, mean, inv_std)
+    fd = mlp.finite_difference_gradient(p, x, t, F, H, mean, inv_std)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-7)
+    assert loss > 0
+
+
+def test_oracle_plants_caught():
+    """A dropped ReLU mask or a transposed W2 changes the gradient (the check above has teeth)."""
+    x, t, p, mean, inv_std = _data()
+    _, g = mlp.loss_and_gradient(p, x, t, F, H, mean, inv_std)
+    W1, b1, W2, b2, w3, b3 = mlp.unpack(p, F, H)
+    q = p.copy()
+    o = mlp.layout(F, H)
+    q[o["W2"]:o["b2"]] = W2.T.ravel()
+    _, gq = mlp.loss_and_gradient(q, x, t, F, H, mean, inv_std)
+    fdq = mlp.finite_difference_gradient(p, x, t, F, H, mean, inv_std)
+    assert not np.allclose(gq, fdq, rtol=1e-3, atol=1e-6)
+
+
+def test_oracle_adam_reduces_loss():
+    x, t, p, mean, inv_std = _data(200)
+    _, losses = mlp.adam_steps(p, x, t, F, H, 300, 1e-2, mean, inv_std)
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_feature_names_match_engine():
+    import re
+    from bohrium_amd import costmodel
+    src = open(os.path.join(ROOT, "bohrium_amd", "csrc", "ve_hip", "features.cpp")).read()
+    block = src[src.index("kFeatureNames[kNumFeatures] = {"):]
+    block = block[:block.index("};")]
+    assert re.findall(r'"(\w+)"', block) == costmodel.FEATURES
+
+
+def test_random_dag_is_deterministic():
+    from bohrium_amd import workloads, _bh_api as api
+
+    def trace(seed):
+        with api.record_only() as r:
+            outs = workloads.random_dag(seed, 120)
+            del outs
+            api.flush()
+        return [(e[0], e[1]) if e[0] == "op" else e[0] for e in r.trace]
+
+    assert trace(3) == trace(3)
+    assert trace(3) != trace(4)
+
+
+# ---------------------------------------------------------------- on the GPU
+
+@pytest.mark.gpu
+def test_hip_forward_and_gradient_match_oracle():
+    from bohrium_amd import costmodel
+    x, t, _, mean, inv_std = _data(300, seed=2)
+    m = costmodel.CostModel(F, H, seed=5)
+    p = m.params().astype(np.float64)
+    m.set_normalization(mean, inv_std)
+    y, _ = mlp.forward(p, x, F, H, mean, inv_std)
+    np.testing.assert_allclose(m.predict(x), y, rtol=1e-4, atol=1e-4)
+    loss, g = m.gradient(x, t)
+    lo, go = mlp.loss_and_gradient(p, x.astype(np.float32), t.astype(np.float32), F, H,
+                                   mean.astype(np.float32), inv_std.astype(np.float32))
+    assert abs(loss - lo) <= 1e-4 * max(1.0, lo)
+    np.testing.assert_allclose(g, go, rtol=2e-3, atol=2e-5 * np.abs(go).max())
+
+
+@pytest.mark.gpu
+def test_hip_adam_training_tracks_oracle():
+    from bohrium_amd import costmodel
+    x, t, _, mean, inv_std = _data(256, seed=3)
+    m = costmodel.CostModel(F, H, seed=7)
+    m.set_normalization(mean, inv_std)
+    p0 = m.params().astype(np.float64)
+    losses = m.train(x, t, 20, lr=1e-2)
+    _, ref = mlp.adam_steps(p0, x.astype(np.float32), t.astype(np.float32), F, H, 20, 1e-2,
+                            mean.astype(np.float32), inv_std.astype(np.float32))
+    np.testing.assert_allclose(losses, ref, rtol=2e-3, atol=1e-5)
+    more = m.train(x, t, 400, lr=1e-2)
+    assert more[-1] < 0.5 * losses[0]
