@@ -20,6 +20,7 @@ const char *strategy_name(Strategy s) {
         case Strategy::FULL_REDUCE: return "reduce";
         case Strategy::SCAN: return "scan";
         case Strategy::SERIAL: return "serial";
+        case Strategy::ROW_REDUCE: return "row";
     }
     return "?";
 }
@@ -366,6 +367,8 @@ private:
     std::set<const bh_base *> index_checked;  // bases needing their element count (gather/scatter)
     // Instructions a skeleton implements itself (collapsed reductions and their identities).
     std::set<const bh_instruction *> skip;
+    // ROW_REDUCE: the inner loop whose iterations are spread over the 64 lanes of a wave.
+    const LoopB *lane_loop = nullptr;
     // Instruction replaced by "store its input into `scan_sink`" in the scan skeleton.
     const bh_instruction *scan_instr = nullptr;
     std::string scan_sink;
@@ -454,6 +457,7 @@ private:
     void grid_stride(const std::string &tot, int unroll, const Element &e);
     std::vector<const bh_view *> preloadable(const LoopB &l);
     void emit_map();
+    void emit_row_reduce();
     void emit_full_reduce();
     void emit_scan();
     void emit_serial();
@@ -569,7 +573,12 @@ void Writer::emit_block(const Block &b, Scope &sc, int ind) {
     }
     const LoopB &c = b.getLoop();
     indent(body, ind);
-    body << "for (int64_t i" << c.rank << " = 0; i" << c.rank << " < n" << lid.at(&c) << "; ++i" << c.rank << ") {\n";
+    if (&c == lane_loop) {
+        body << "for (int64_t i" << c.rank << " = lane_; i" << c.rank << " < n" << lid.at(&c) << "; i" << c.rank
+             << " += 64) {\n";
+    } else {
+        body << "for (int64_t i" << c.rank << " = 0; i" << c.rank << " < n" << lid.at(&c) << "; ++i" << c.rank << ") {\n";
+    }
     emit_body(c, &sc, ind + 1);
     indent(body, ind);
     body << "}\n";
@@ -764,6 +773,97 @@ void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
     e.decode(2);
     e.compute(2, e.outer);
     body << "  }\n";
+}
+
+// The inner loop of a row reduction: the innermost parallel loop of `l0` holds only constant
+// IDENTITY seeds of the reductions of one innermost sub-loop (and frees), and that sub-loop's
+// sweeps are all reductions over its own rank.  Null when `l0` has another form.
+const LoopB *row_inner_loop(const LoopB &l0) {
+    const uint64_t p = std::max<uint64_t>(1, jitk::parallel_ranks(l0).first);
+    const std::vector<const LoopB *> chain = jitk::get_first_loop_blocks(l0);
+    if (chain.size() < p) return nullptr;
+    const LoopB &P = *chain[p - 1];
+    const LoopB *C = nullptr;
+    for (const Block &b : P._block_list)
+        if (!b.isInstr()) {
+            if (C != nullptr) return nullptr;
+            C = &b.getLoop();
+        }
+    if (C == nullptr || !C->isInnermost() || C->_sweeps.empty() || C->size < 32) return nullptr;
+    std::set<const bh_base *> outs;
+    for (const InstrPtr &sw : C->_sweeps) {
+        if (!bh_opcode_is_reduction(sw->opcode) || sw->sweep_axis() != C->rank ||
+            sw->sweep_axis() != sw->operand[1].ndim - 1)
+            return nullptr;
+        outs.insert(sw->operand[0].base);
+    }
+    for (const Block &b : P._block_list) {
+        if (!b.isInstr()) continue;
+        const InstrPtr &i = b.getInstr();
+        if (bh_opcode_is_system(i->opcode)) continue;
+        if (i->opcode != BH_IDENTITY || !i->operand[1].isConstant() || !outs.count(i->operand[0].base)) return nullptr;
+    }
+    // The reduced values may only be stored (no other instruction of the block reads them).
+    for (const InstrPtr &i : jitk::allInstr(l0))
+        for (size_t o = 1; o < i->operand.size(); ++o)
+            if (!i->operand[o].isConstant() && outs.count(i->operand[o].base)) return nullptr;
+    return C;
+}
+
+// ---- ROW_REDUCE: one wave per row of the flattened parallel ranks
+void Writer::emit_row_reduce() {
+    const LoopB &l0 = unit._block_list.back().getLoop();
+    const uint64_t p = std::max<uint64_t>(1, jitk::parallel_ranks(l0).first);
+    const std::vector<const LoopB *> chain = jitk::get_first_loop_blocks(l0);
+    const LoopB *C = row_inner_loop(l0);
+    if (C == nullptr) throw std::logic_error("row reduction: block has another form");
+    lane_loop = C;
+    uint64_t rows = 1;
+    for (uint64_t r = 0; r < p; ++r) rows *= static_cast<uint64_t>(chain[r]->size);
+    kp.work_items = rows * static_cast<uint64_t>(C->size);
+    Scope ksc;
+    declare_scope(unit, ksc, 1);
+    body << "  const int64_t rows_ = 1";
+    for (uint64_t r = 0; r < p; ++r) body << " * n" << lid.at(chain[r]);
+    body << ";\n";
+    body << "  const int lane_ = threadIdx.x & 63;\n";
+    body << "  for (int64_t f_ = (int64_t)bh_bid * BH_WAVES + (threadIdx.x >> 6); f_ < rows_; f_ += (int64_t)bh_nblk * BH_WAVES) {\n";
+    body << "    int64_t r_ = f_;\n";
+    for (int64_t r = static_cast<int64_t>(p) - 1; r >= 1; --r)
+        body << "    const int64_t i" << r << " = r_ % n" << lid.at(chain[r]) << "; r_ /= n" << lid.at(chain[r]) << ";\n";
+    body << "    const int64_t i0 = r_;\n";
+    std::vector<Scope> scopes(p);
+    const Scope *parent = &ksc;
+    for (uint64_t r = 0; r < p; ++r) {
+        scopes[r].parent = parent;
+        body << std::string(4 + 2 * r, ' ') << "{\n";
+        declare_scope(*chain[r], scopes[r], 3 + static_cast<int>(r));
+        parent = &scopes[r];
+    }
+    Scope &ps = scopes[p - 1];
+    const int ind = 2 + static_cast<int>(p);
+    emit_blocks(*chain[p - 1], ps, ind);
+    // Fold the lanes' partial reductions; every lane ends with the row's value.
+    for (const InstrPtr &sw : C->_sweeps) {
+        const bh_view &v = sw->operand[0];
+        const std::string var = ref(ps, v, static_cast<int>(sw->sweep_axis()));
+        const std::string T = hip_type(v.base->dtype());
+        indent(body, ind);
+        body << var << " = bh_wave_reduce(" << var << ", [](" << T << " x, " << T << " y) -> " << T << " { return "
+             << combine_expr(sw->opcode, "x", "y") << "; });\n";
+    }
+    indent(body, ind);
+    body << "if (lane_ == 0) {\n";
+    write_back(*chain[p - 1], ps, ind + 1);
+    indent(body, ind);
+    body << "}\n";
+    for (int64_t r = static_cast<int64_t>(p) - 1; r >= 0; --r) {
+        if (r < static_cast<int64_t>(p) - 1) write_back(*chain[r], scopes[r], 3 + static_cast<int>(r));
+        body << std::string(4 + 2 * r, ' ') << "}\n";
+    }
+    body << "  }\n";
+    write_back(unit, ksc, 1);
+    kp.grid = std::max<uint64_t>(1, std::min<uint64_t>((rows + 3) / 4, cfg.max_grid));
 }
 
 // ---- MAP: grid-stride over the flattened leading parallel ranks
@@ -1161,6 +1261,7 @@ KernelPlan Writer::plan(Strategy s) {
     if (s == Strategy::SERIAL) kp.block = 64;
     switch (s) {
         case Strategy::MAP: emit_map(); break;
+        case Strategy::ROW_REDUCE: emit_row_reduce(); break;
         case Strategy::FULL_REDUCE: emit_full_reduce(); break;
         case Strategy::SCAN: emit_scan(); break;
         case Strategy::SERIAL: emit_serial(); break;
@@ -1401,6 +1502,7 @@ std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg
         LoopB unit(-1, 1, scalar_ids);
         unit._block_list.push_back(b);
         if (s == Strategy::MAP && unit._block_list.size() > 1) s = Strategy::SERIAL;
+        if (s == Strategy::MAP && cfg.row_reduce && row_inner_loop(l0) != nullptr) s = Strategy::ROW_REDUCE;
         if (s == Strategy::FULL_REDUCE) {
             while (bi + 1 < blocks.size() && (blocks[bi + 1].isSystemOnly() || tail_block(blocks[bi + 1])))
                 unit._block_list.push_back(blocks[++bi]);

@@ -10,6 +10,9 @@
  *                and runs the small scalar blocks that follow in thread 0.
  *   SCAN         rank-0 add/multiply accumulate: wave64 shuffle scan, LDS
  *                across waves and a decoupled look-back across workgroups.
+ *   ROW_REDUCE   a map whose innermost parallel loop only seeds and reduces an inner
+ *                loop (row sums of an expression): one wave per row, lanes stride the
+ *                row (coalesced), wave64 butterfly, lane 0 stores.
  *   SERIAL       anything else, executed by one thread (always correct).
  * Per-opcode scalar semantics restate the reference's write_operation
  * (core/jitk/instruction.cpp:58-544) and engine.cpp:251-336 special forms.
@@ -25,7 +28,7 @@
 namespace bohrium {
 namespace vehip {
 
-enum class Strategy { MAP, FULL_REDUCE, SCAN, SERIAL };
+enum class Strategy { MAP, FULL_REDUCE, SCAN, SERIAL, ROW_REDUCE };
 const char *strategy_name(Strategy s);
 
 /* One kernel argument: either literal bytes known at plan time, or a pointer
@@ -89,6 +92,7 @@ struct WriterConfig {
     int reduce_min_items = 4;    // FULL_REDUCE: elements per thread before another workgroup is added
     int scan_items = 8;      // elements per thread per scan tile
     int map_unroll = 1;      // grid-stride elements per thread issued together (MAP)
+    bool row_reduce = true;  // run row reductions with one wave per row (ROW_REDUCE)
     int reduce_unroll = 1;   // the same for FULL_REDUCE
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
