@@ -453,6 +453,9 @@ private:
         std::function<void(int, const Scope *)> compute;
         const LoopB *body_loop = nullptr;  // innermost loop whose read-only inputs may be preloaded
         const Scope *outer = nullptr;
+        /* A 2-D iteration (i0 outer, i1 inner of this extent): without unrolling, the two indices
+         * are advanced by the grid stride instead of being divided out of f_ for every element. */
+        std::string inner;
     };
     void grid_stride(const std::string &tot, int unroll, const Element &e);
     std::vector<const bh_view *> preloadable(const LoopB &l);
@@ -768,6 +771,17 @@ void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
         }
         body << "  }\n";
     }
+    if (unroll <= 1 && !e.inner.empty()) {
+        const std::string &n = e.inner;
+        body << "  int64_t i0_ = g_ / " << n << ", i1_ = g_ - i0_ * " << n << ";\n";
+        body << "  const int64_t sq_ = st_ / " << n << ", sr_ = st_ - sq_ * " << n << ";\n";
+        body << "  for (; g_ < " << tot << "; g_ += st_) {\n";
+        body << "    const int64_t i0 = i0_, i1 = i1_;\n";
+        e.compute(2, e.outer);
+        body << "    i1_ += sr_;\n    i0_ += sq_;\n    if (i1_ >= " << n << ") { i1_ -= " << n << "; ++i0_; }\n";
+        body << "  }\n";
+        return;
+    }
     body << "  for (; g_ < " << tot << "; g_ += st_) {\n";
     body << "    const int64_t f_ = g_;\n";
     e.decode(2);
@@ -885,6 +899,7 @@ void Writer::emit_map() {
     Element e;
     e.outer = &ksc;
     e.body_loop = flat ? chain[p - 1] : nullptr;
+    if (p == 2) e.inner = "n" + std::to_string(lid.at(chain[1]));
     e.decode = [&](int ind) {
         const std::string pad(2 * ind, ' ');
         if (p == 1) {
@@ -1064,6 +1079,7 @@ void Writer::emit_full_reduce() {
         Element e;
         e.outer = &ksc;
         e.body_loop = l1;
+        e.inner = n1;
         e.decode = [&](int ind) {
             const std::string pad(2 * ind, ' ');
             if (total < (1ull << 31)) {
@@ -1306,7 +1322,12 @@ KernelPlan Writer::plan(Strategy s) {
     if (s == Strategy::FULL_REDUCE) dev << "\n__device__ __forceinline__ void bh_fin_(" << plist << ") {\n" << body.str() << "}\n";
     kp.device_code = dev.str();
     std::ostringstream src;
-    src << kp.device_code << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.name << "("
+    // Occupancy hint for the main kernel (waves per SIMD the register allocator must allow).
+    kp.attributes = cfg.waves_per_eu > 0 && s != Strategy::SERIAL
+                        ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(cfg.waves_per_eu) + ")))"
+                        : "";
+    src << kp.device_code << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ")" << kp.attributes
+        << " " << kp.name << "("
         << plist << ") {\n  bh_main_(" << nlist << (nlist.empty() ? "" : ", ") << "blockIdx.x, gridDim.x);\n}\n";
     if (s == Strategy::FULL_REDUCE)
         src << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.fin_name << "(" << plist
@@ -1546,7 +1567,7 @@ std::string batch_source(const KernelPlan &p) {
     s << p.device_code;
     // Workgroup b of the batched grid runs workgroup b - first[i] of item i, where
     // first[i] <= b < first[i + 1]; each item's packed arguments sit at args + i * stride.
-    s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ") bh_kernel_batch("
+    s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ")" << p.attributes << " bh_kernel_batch("
       << "const unsigned char *__restrict__ bh_args, const unsigned *__restrict__ bh_first, const unsigned bh_items, "
          "const unsigned bh_stride) {\n"
       << "  unsigned lo_ = 0, hi_ = bh_items;\n"

@@ -64,6 +64,7 @@ struct KernelPlan {
     /* The body as device functions (bh_main_, and bh_fin_ for FULL_REDUCE) and their parameters,
      * one per ArgSpec: declaration, value type and name.  `source` adds the kernels calling them. */
     std::string device_code;
+    std::string attributes;  // extra attributes of the main kernel (occupancy hint)
     std::vector<std::string> param_decls, param_types, param_names;
     uint64_t grid = 1;
     uint32_t block = 256;
@@ -93,6 +94,7 @@ struct WriterConfig {
     int scan_items = 8;      // elements per thread per scan tile
     int map_unroll = 1;      // grid-stride elements per thread issued together (MAP)
     bool row_reduce = true;  // run row reductions with one wave per row (ROW_REDUCE)
+    int waves_per_eu = 0;    // >0: amdgpu_waves_per_eu occupancy hint on the main kernels
     int reduce_unroll = 1;   // the same for FULL_REDUCE
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };

@@ -124,6 +124,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
     wcfg.reduce_unroll = std::max(1, cfg.defaultGet<int>("reduce_unroll", 1));
     wcfg.row_reduce = cfg.defaultGet<bool>("row_reduce", true);
+    wcfg.waves_per_eu = std::max(0, cfg.defaultGet<int>("waves_per_eu", 0));
     unflatten = cfg.defaultGet<bool>("unflatten_reductions", true);
     resident = cfg.defaultGet<bool>("resident", true);
     graph_repeats = cfg.defaultGet<bool>("graph_repeats", true);
@@ -784,7 +785,7 @@ public:
                 if (eq == std::string::npos) throw std::invalid_argument("hip engine: bad codegen setting: " + kv);
                 const std::string k = kv.substr(0, eq);
                 const int v = std::stoi(kv.substr(eq + 1));
-                if (v < 0 || (v < 1 && k != "row_reduce"))
+                if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu"))
                     throw std::invalid_argument("hip engine: codegen setting must be >= 1: " + kv);
                 if (k == "max_grid") {
                     engine.codegenConfig().max_grid = v;
@@ -800,6 +801,8 @@ public:
                     engine.codegenConfig().scan_items = v;
                 } else if (k == "row_reduce") {
                     engine.codegenConfig().row_reduce = v != 0;
+                } else if (k == "waves_per_eu") {
+                    engine.codegenConfig().waves_per_eu = v;
                 } else {
                     throw std::invalid_argument("hip engine: unknown codegen setting: " + k);
                 }
@@ -807,7 +810,8 @@ public:
             const WriterConfig &w = engine.codegenConfig();
             ss << "max_grid=" << w.max_grid << " map_unroll=" << w.map_unroll << " reduce_unroll=" << w.reduce_unroll
                << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items
-               << " scan_items=" << w.scan_items << " row_reduce=" << (w.row_reduce ? 1 : 0);
+               << " scan_items=" << w.scan_items << " row_reduce=" << (w.row_reduce ? 1 : 0)
+               << " waves_per_eu=" << w.waves_per_eu;
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
