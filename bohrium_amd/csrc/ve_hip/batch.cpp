@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cstring>
 #include <stdexcept>
+#include <set>
 #include <unordered_map>
 
 #include "../../../include/bhhip.h"
@@ -133,6 +134,48 @@ void EngineHip::runSchedule(CachedFlush &cf, const std::vector<bh_base *> &ord2b
         for (size_t g : lv.groups) launchBatch(*s.groups[g], cf, ord2base, instrs);
         for (uint32_t k : lv.free_kernels)
             for (int o : cf.kernels[k].free_ordinals) release(ord2base[o]);
+    }
+}
+
+// Compile-only mode with BH_HIP_DUMP_DIR: one file per batched launch with the batch source, the
+// first-workgroup table and every item's packed arguments (bases as ordinal + 1 shifted by 32 bits),
+// so the batched launch can be replayed on the host with checked arrays.
+void EngineHip::dumpSchedule(CachedFlush &cf, const std::vector<bh_base *> &ord2base,
+                             const std::vector<bh_instruction *> &instrs, const char *dir) {
+    static uint64_t seq = 0;
+    for (const auto &gp : cf.schedule->groups) {
+        const BatchGroup &g = *gp;
+        const KernelPlan &p0 = cf.kernels[g.items[0].kernel].plans[g.items[0].plan];
+        size_t packed = 0;
+        arg_offsets(p0.args, &packed);
+        const uint64_t stride = round_up(packed, 16);
+        const std::string path = std::string(dir) + "/batch_" + std::to_string(seq++) + ".txt";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (f == nullptr) throw std::runtime_error("hip engine: cannot write " + path);
+        std::fprintf(f, "strategy %s\nitems %zu\nblock %u\nstride %llu\nfirst", strategy_name(p0.strategy),
+                     g.items.size(), p0.block, static_cast<unsigned long long>(stride));
+        uint64_t blocks = 0;
+        for (const PlanRef &r : g.items) {
+            std::fprintf(f, " %llu", static_cast<unsigned long long>(blocks));
+            blocks += cf.kernels[r.kernel].plans[r.plan].grid;
+        }
+        std::fprintf(f, " %llu\n", static_cast<unsigned long long>(blocks));
+        std::vector<uint8_t> buf(stride);
+        std::set<int> ords;
+        for (const PlanRef &r : g.items) {
+            const KernelPlan &p = cf.kernels[r.kernel].plans[r.plan];
+            std::fill(buf.begin(), buf.end(), 0);
+            packArgs(p, ord2base, instrs, nullptr, buf.data(), stride);
+            std::fprintf(f, "item");
+            for (uint8_t b : buf) std::fprintf(f, " %u", b);
+            std::fprintf(f, "\n");
+            ords.insert(p.param_ordinals.begin(), p.param_ordinals.end());
+        }
+        for (int o : ords)
+            std::fprintf(f, "base %d %lld %d\n", o + 1, static_cast<long long>(ord2base[o]->nelem()),
+                         bh_type_size(ord2base[o]->dtype()));
+        std::fprintf(f, "code\n%s", batch_source(p0).c_str());
+        std::fclose(f);
     }
 }
 

@@ -1482,6 +1482,58 @@ std::vector<LoopB> fold_serial_tails(std::vector<LoopB> kernels) {
     return out;
 }
 
+namespace {
+// Every operand dimension of every instruction must be iterated by the enclosing loop of the
+// matching rank, and that loop may not run past the dimension (the generated index is
+// start + sum(i_rank * stride), in bounds only then).  `sizes` holds the enclosing loop sizes
+// by rank.
+void verify_loop_ranks(const LoopB &l, std::vector<int64_t> &sizes, std::string &err) {
+    if (l.rank >= 0) {
+        if (static_cast<int64_t>(sizes.size()) != l.rank) {
+            err += "loop rank " + std::to_string(l.rank) + " nested at depth " + std::to_string(sizes.size()) + "; ";
+            return;
+        }
+        sizes.push_back(l.size);
+    }
+    for (const Block &b : l._block_list) {
+        if (!b.isInstr()) {
+            verify_loop_ranks(b.getLoop(), sizes, err);
+            continue;
+        }
+        const InstrPtr &i = b.getInstr();
+        if (bh_opcode_is_system(i->opcode)) continue;
+        for (size_t o = 0; o < i->operand.size(); ++o) {
+            const bh_view &v = i->operand[o];
+            if (v.isConstant()) continue;
+            if (i->opcode == BH_GATHER && o == 1) continue;            // indexed by value, checked at run time
+            if ((i->opcode == BH_SCATTER || i->opcode == BH_COND_SCATTER) && o == 0) continue;
+            int hidden = BH_MAXDIM;
+            if (o == 0 && bh_opcode_is_reduction(i->opcode) && i->operand[1].ndim > 1) hidden = i->sweep_axis();
+            if (o == 0 && bh_opcode_is_reduction(i->opcode) && i->operand[1].ndim <= 1) continue;  // scalar output
+            for (int64_t d = 0; d < v.ndim; ++d) {
+                const int64_t t = d >= hidden ? d + 1 : d;
+                // an extent-1 axis only ever takes index 0 unless a loop of several iterations walks it
+                if (v.shape[d] <= 1 && (v.stride[d] == 0 || t >= static_cast<int64_t>(sizes.size()) || sizes[t] <= 1))
+                    continue;
+                if (t >= static_cast<int64_t>(sizes.size()) || sizes[t] > v.shape[d]) {
+                    err += std::string(bh_opcode_text(i->opcode)) + " operand " + std::to_string(o) + " dim " +
+                           std::to_string(d) + " (extent " + std::to_string(v.shape[d]) + ") iterated by rank " +
+                           std::to_string(t) + (t < static_cast<int64_t>(sizes.size()) ? " of size " + std::to_string(sizes[t]) : " (none)") + "; ";
+                }
+            }
+        }
+    }
+    if (l.rank >= 0) sizes.pop_back();
+}
+} // namespace
+
+std::string verify_kernel(const LoopB &kernel) {
+    std::vector<int64_t> sizes;
+    std::string err;
+    verify_loop_ranks(kernel, sizes, err);
+    return err;
+}
+
 std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg) {
     std::vector<KernelPlan> ret;
     auto emit = [&](LoopB unit, Strategy s) {

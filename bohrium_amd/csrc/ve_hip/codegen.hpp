@@ -103,6 +103,10 @@ struct WriterConfig {
  * that one ends in a FULL_REDUCE loop: its finalize kernel then runs them. */
 std::vector<jitk::LoopB> fold_serial_tails(std::vector<jitk::LoopB> kernels);
 
+/* Check that every operand dimension is iterated within its extent by the loop of the matching
+ * rank (what the generated indexing assumes); returns "" or a description of each violation. */
+std::string verify_kernel(const jitk::LoopB &kernel);
+
 /* Split a jitk kernel (rank -1 LoopB) into launch plans. */
 std::vector<KernelPlan> plan_kernel(const jitk::LoopB &kernel, const WriterConfig &cfg);
 

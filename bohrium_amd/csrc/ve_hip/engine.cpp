@@ -237,6 +237,8 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
                               std::to_string(plan.block) + " " + hex);
         ++stats.launches;
         ++stats.launches_by_strategy[strategy_name(plan.strategy)];
+        static const char *dump = std::getenv("BH_HIP_DUMP_DIR");
+        if (dump != nullptr && *dump) dumpLaunch(dump, plan, ord2base, instrs);
         return;
     }
     if (plan.function == nullptr) {
@@ -281,6 +283,48 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
     stats.time_launch += seconds_since(t);
 }
 
+// Compile-only mode with BH_HIP_DUMP_DIR: one file per planned launch with its device code, shape and
+// the resolved value of every argument (bases as element count and element size), so the launch can
+// be replayed on the host by a CPU build of the same code.
+void EngineHip::dumpLaunch(const char *dir, const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
+                           const std::vector<bh_instruction *> &instrs) {
+    static uint64_t seq = 0;
+    const std::string path = std::string(dir) + "/launch_" + std::to_string(seq++) + ".txt";
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (f == nullptr) throw std::runtime_error("hip engine: cannot write " + path);
+    std::fprintf(f, "strategy %s\ngrid %llu\nblock %u\nscratch %lld\n", strategy_name(plan.strategy),
+                 static_cast<unsigned long long>(plan.grid), plan.block, static_cast<long long>(plan.scratch_bytes));
+    for (size_t i = 0; i < plan.args.size(); ++i) {
+        const ArgSpec &a = plan.args[i];
+        std::fprintf(f, "param %s %s ", plan.param_types[i].c_str(), plan.param_names[i].c_str());
+        if (a.kind == ArgSpec::BASE_PTR) {
+            const bh_base *b = ord2base[a.base_ordinal];
+            std::fprintf(f, "base %lld %d\n", static_cast<long long>(b->nelem()), bh_type_size(b->dtype()));
+        } else if (a.kind != ArgSpec::LITERAL) {
+            std::fprintf(f, "engine\n");
+        } else {
+            uint8_t buf[64];
+            const uint8_t *src = a.bytes.data();
+            int64_t start = 0;
+            if (a.view_origin >= 0 && a.view_origin < static_cast<int64_t>(instrs.size())) {
+                start = instrs[a.view_origin]->operand[a.view_operand].start;
+                src = reinterpret_cast<const uint8_t *>(&start);
+            } else if (a.const_origin >= 0 && a.const_origin < static_cast<int64_t>(instrs.size())) {
+                const bh_constant &c = instrs[a.const_origin]->constant;
+                src = a.const_part == 0 ? reinterpret_cast<const uint8_t *>(&c.value.r123.start)
+                    : a.const_part == 1 ? reinterpret_cast<const uint8_t *>(&c.value.r123.key)
+                                        : reinterpret_cast<const uint8_t *>(&c.value);
+            }
+            std::memcpy(buf, src, static_cast<size_t>(a.size));
+            std::fprintf(f, "bytes");
+            for (int k = 0; k < a.size; ++k) std::fprintf(f, " %u", buf[k]);
+            std::fprintf(f, "\n");
+        }
+    }
+    std::fprintf(f, "code\n%s", plan.device_code.c_str());
+    std::fclose(f);
+}
+
 void EngineHip::compilePlan(KernelPlan &plan) {
     check(bhhip_compile(kPrelude, plan.source.c_str(), plan.name.c_str(), cache_dir.c_str(), &plan.function),
           "kernel compilation");
@@ -317,7 +361,11 @@ size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> 
                     put(a.bytes.data(), a.size, a.align);
                 }
                 continue;
-            case ArgSpec::BASE_PTR: ptr = bufs.at(ord2base[a.base_ordinal]).ptr; break;
+            case ArgSpec::BASE_PTR:
+                // (compile-only dumps stand a base's ordinal + 1, shifted, in for its address)
+                ptr = dry_run ? reinterpret_cast<void *>(static_cast<uintptr_t>(a.base_ordinal + 1) << 32)
+                              : bufs.at(ord2base[a.base_ordinal]).ptr;
+                break;
             case ArgSpec::SCRATCH_PTR: ptr = scratch; break;
             case ArgSpec::COUNTER_PTR: ptr = counters; break;
             case ArgSpec::ERRFLAG_PTR: ptr = counters + 1; break;
@@ -466,6 +514,10 @@ void EngineHip::handleExecution(BhIR *bhir) {
             if (debug) std::fprintf(stderr, "[hip] kernel:\n%s", kernel.pprint().c_str());
             CachedKernel ck;
             if (!kernel.isSystemOnly()) {
+                const std::string bad = verify_kernel(kernel);
+                if (!bad.empty())
+                    throw std::logic_error("hip engine: kernel loops do not match its views: " + bad + "\n" +
+                                           kernel.pprint());
                 ck.plans = plan_kernel(kernel, wcfg);
                 const std::vector<float> feats = block_features(jitk::allInstr(kernel));
                 for (KernelPlan &p : ck.plans) p.features = feats;
@@ -498,6 +550,8 @@ void EngineHip::handleExecution(BhIR *bhir) {
     }
     CachedFlush &cf = hit->second;
     if (!cf.schedule_built) buildSchedule(cf);
+    static const char *dump_dir = std::getenv("BH_HIP_DUMP_DIR");
+    if (cf.schedule && dry_run && dump_dir != nullptr && *dump_dir) dumpSchedule(cf, ord2base, instr_list, dump_dir);
     if (cf.schedule && !dry_run) {
         runSchedule(cf, ord2base, instr_list);
     } else {

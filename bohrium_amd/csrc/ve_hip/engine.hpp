@@ -189,11 +189,15 @@ private:
     };
     void buildSchedule(CachedFlush &cf);
     void runSchedule(CachedFlush &cf, const std::vector<bh_base *> &ord2base, const std::vector<bh_instruction *> &instrs);
+    void dumpSchedule(CachedFlush &cf, const std::vector<bh_base *> &ord2base, const std::vector<bh_instruction *> &instrs,
+                      const char *dir);
     void launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh_base *> &ord2base,
                      const std::vector<bh_instruction *> &instrs);
     size_t packArgs(const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
                     const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap);
     void compilePlan(KernelPlan &plan);
+    void dumpLaunch(const char *dir, const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
+                    const std::vector<bh_instruction *> &instrs);
     std::unordered_map<std::string, CachedFlush> flush_cache;
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
     bool last_index_checks = false;
