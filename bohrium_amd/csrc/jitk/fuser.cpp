@@ -502,6 +502,9 @@ std::vector<InstrPtr> swap_axis(const std::vector<InstrPtr> &list, int64_t a1, i
     return ret;
 }
 
+// The parent is split into one loop per sub-block; the swapped child is rebuilt from its
+// instructions, so every free of its subtree is kept on the rebuilt loop, and the parent's
+// own frees go to the last loop of the split (after every use inside the parent).
 std::vector<Block> swap_blocks(const LoopB &parent, const LoopB *child) {
     std::vector<Block> ret;
     for (const Block &b : parent._block_list) {
@@ -510,16 +513,16 @@ std::vector<Block> swap_blocks(const LoopB &parent, const LoopB *child) {
         if (b.isInstr() || &b.getLoop() != child) {
             loop.size = parent.size;
             loop._block_list.push_back(b);
-            loop._frees.insert(parent._frees.begin(), parent._frees.end());
         } else {
             loop.size = child->size;
             const auto t = swap_axis(allInstr(*child), parent.rank, child->rank);
             loop._block_list.push_back(create_nested_block(t, child->rank, parent.size));
-            loop._frees.insert(child->_frees.begin(), child->_frees.end());
+            loop._frees = child->getAllFrees();
         }
-        loop.metadataUpdate();
         ret.emplace_back(std::move(loop));
     }
+    if (!ret.empty()) ret.back().getLoop()._frees.insert(parent._frees.begin(), parent._frees.end());
+    for (Block &b : ret) b.getLoop().metadataUpdate();
     return ret;
 }
 

@@ -1572,9 +1572,14 @@ std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg
         }
         pending.clear();
         if (s == Strategy::MAP && l0.size == 0) continue;
+        // Scalar writers in front of a map run first as their own one-thread launch, so the
+        // map keeps its grid instead of running serially behind them.
+        if (s == Strategy::MAP && !scalar_ids.empty()) {
+            emit(LoopB(-1, 1, scalar_ids), Strategy::SERIAL);
+            scalar_ids.clear();
+        }
         LoopB unit(-1, 1, scalar_ids);
         unit._block_list.push_back(b);
-        if (s == Strategy::MAP && unit._block_list.size() > 1) s = Strategy::SERIAL;
         if (s == Strategy::MAP && cfg.row_reduce && row_inner_loop(l0) != nullptr) s = Strategy::ROW_REDUCE;
         if (s == Strategy::FULL_REDUCE) {
             while (bi + 1 < blocks.size() && (blocks[bi + 1].isSystemOnly() || tail_block(blocks[bi + 1])))

@@ -156,7 +156,15 @@ EngineHip::~EngineHip() {
 
 DeviceBuffer &EngineHip::buffer(bh_base *base) {
     auto it = bufs.find(base);
-    if (it != bufs.end()) return it->second;
+    if (it != bufs.end()) {
+        // Buffers are filed by base address; one of another size means a freed base was never
+        // released and a new base took its address.  Launching with it would write out of bounds.
+        const DeviceBuffer &d = it->second;
+        if (d.ptr != nullptr && !d.external && d.bytes != static_cast<uint64_t>(base->nbytes()))
+            throw std::logic_error("hip engine: a device buffer of " + std::to_string(d.bytes) +
+                                   " bytes is filed under a base of " + std::to_string(base->nbytes()) + " bytes");
+        return it->second;
+    }
     DeviceBuffer &b = bufs[base];
     b.host_valid = base->getDataPtr() != nullptr;
     return b;
@@ -200,6 +208,7 @@ void EngineHip::copyAllToHost() {
 }
 
 void EngineHip::release(bh_base *base) {
+    released_now.insert(base);
     auto it = bufs.find(base);
     if (it != bufs.end()) {
         if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
@@ -491,6 +500,7 @@ void EngineHip::checkErrors() {
 void EngineHip::handleExecution(BhIR *bhir) {
     const auto t_total = std::chrono::steady_clock::now();
     ++stats.flushes;
+    released_now.clear();
     std::set<bh_base *> frees;
     std::vector<bh_instruction *> instr_list = jitk::remove_non_computed_system_instr(bhir->instr_list, frees);
     for (bh_base *b : frees) release(b);
@@ -561,6 +571,15 @@ void EngineHip::handleExecution(BhIR *bhir) {
             for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
             for (int o : ck.free_ordinals) release(ord2base[o]);
         }
+    }
+    // The bridge deletes a freed base after this flush and may reuse its address for a new base;
+    // a buffer still filed under that address would then be handed to the new base.
+    for (const bh_instruction &i : bhir->instr_list) {
+        if (i.opcode != BH_FREE || released_now.count(i.operand[0].base)) continue;
+        ++missed_frees;
+        if (debug) std::fprintf(stderr, "[hip] BH_FREE of a%p was not released by its kernel\n",
+                                static_cast<void *>(i.operand[0].base));
+        release(i.operand[0].base);
     }
     const bool index_checks = cf.index_checks;
     last_index_checks = index_checks;
@@ -676,6 +695,7 @@ std::string EngineHip::statistics() const {
        << "\ntime_copy: " << stats.time_copy << "\ntime_total: " << stats.time_total << "\n";
     for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
     ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
+    ss << "missed_frees: " << missed_frees << "\n";
     ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
     ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";

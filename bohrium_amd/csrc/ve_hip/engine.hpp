@@ -140,6 +140,8 @@ private:
     std::string cache_dir;
     bool resident = true;
     std::unordered_map<const bh_base *, DeviceBuffer> bufs;
+    /* Bases released during the current flush; a BH_FREE that no kernel released is caught with it. */
+    std::set<const bh_base *> released_now;
     DevicePool pool;
     unsigned char *reduce_scratch = nullptr;
     uint64_t reduce_scratch_bytes = 0;
@@ -200,6 +202,7 @@ private:
                     const std::vector<bh_instruction *> &instrs);
     std::unordered_map<std::string, CachedFlush> flush_cache;
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
+    uint64_t missed_frees = 0;
     bool last_index_checks = false;
     uint64_t scratch_allocs = 0;
     std::map<std::string, void *> graph_cache;
