@@ -14,6 +14,7 @@ from . import opcodes
 from .bharray import (BhArray, array, empty, full, zeros, ones, arange, take, put, flush, ufunc,
                       assign, asnumpy_many, Iterator, get_iterator)
 from . import random
+from . import user_kernel
 
 _GLOBAL_UFUNCS = ["add", "subtract", "multiply", "true_divide", "divide", "floor_divide", "power", "absolute",
                   "greater", "greater_equal", "less", "less_equal", "equal", "not_equal", "logical_and",

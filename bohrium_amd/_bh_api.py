@@ -74,6 +74,9 @@ def _load():
     lib.bhc_message.argtypes = [ctypes.c_char_p]
     lib.bhc_message.restype = ctypes.c_char_p
     lib.bhc_extmethod.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(BhcView)]
+    lib.bhc_user_kernel.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(BhcView), ctypes.c_char_p,
+                                    ctypes.c_char_p, ctypes.c_char_p]
+    lib.bhc_user_kernel.restype = ctypes.c_char_p
     lib.bhc_set_flush_threshold.argtypes = [i64]
     lib.bhc_op_slide.argtypes = [i32, i64, i64, i64, i64, i64, i64]
     lib.bhc_base_nelem.argtypes = [vp]
@@ -324,6 +327,19 @@ def extmethod(name, operands):
         arr[i] = make_view(h, start, shape, stride)
     _check(lib().bhc_extmethod(name.encode(), n, arr))
     _rec("ext", name, [(_bid(h), int(s), tuple(sh), tuple(st)) for (h, s, sh, st) in operands])
+
+
+def user_kernel(kernel, operands, compile_cmd, tag, param):
+    """Run a user kernel on `operands` ((handle, start, shape, stride) views); returns the component's
+    message ("" on success)."""
+    n = len(operands)
+    arr = (BhcView * max(n, 1))()
+    for i, (h, start, shape, stride) in enumerate(operands):
+        arr[i] = make_view(h, start, shape, stride)
+    r = lib().bhc_user_kernel(kernel.encode(), n, arr, compile_cmd.encode(), tag.encode(), param.encode())
+    if r is None:
+        raise BohriumError(_lib.bhc_last_error().decode(errors="replace"))
+    return r.decode(errors="replace")
 
 
 def set_flush_threshold(n):

@@ -251,6 +251,25 @@ const char *bhc_message(const char *msg) {
     return ret;
 }
 
+const char *bhc_user_kernel(const char *kernel, int nops, const bhc_view_t *operands, const char *compile_cmd,
+                            const char *tag, const char *param) {
+    static std::string ret_msg;
+    const char *ret = nullptr;
+    guarded([&] {
+        if (kernel == nullptr || tag == nullptr) throw std::invalid_argument("bhc_user_kernel: null kernel or tag");
+        if (nops < 0 || (nops > 0 && operands == nullptr)) throw std::invalid_argument("bhc_user_kernel: bad operands");
+        std::vector<bh_view> ops;
+        for (int i = 0; i < nops; ++i) {
+            ops.push_back(to_view(operands[i]));
+            check_bounds(ops.back());
+        }
+        ret_msg = rt().stack.userKernel(kernel, ops, compile_cmd == nullptr ? "" : compile_cmd, tag,
+                                        param == nullptr ? "" : param);
+        ret = ret_msg.c_str();
+    });
+    return ret;
+}
+
 int bhc_extmethod(const char *name, int nops, const bhc_view_t *views) {
     return guarded([&] {
         Runtime &r = rt();

@@ -270,6 +270,18 @@ int bhhip_launch(void *function, uint64_t grid, uint32_t block, const void *args
     return 0;
 }
 
+int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block[3], const void *args,
+                    size_t args_bytes) {
+    size_t size = args_bytes;
+    void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, const_cast<void *>(args), HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                      HIP_LAUNCH_PARAM_END};
+    if (grid[0] == 0 || grid[1] == 0 || grid[2] == 0) return 0;
+    HIPCHK(hipModuleLaunchKernel(reinterpret_cast<hipFunction_t>(function), grid[0], grid[1], grid[2], block[0],
+                                 block[1], block[2], 0, g_stream, nullptr, config),
+           "hipModuleLaunchKernel");
+    return 0;
+}
+
 int bhhip_synchronize(void) {
     HIPCHK(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
     return 0;

@@ -649,6 +649,91 @@ void *EngineHip::getMemoryPointer(bh_base &base, bool copy2host, bool force_allo
     return ret;
 }
 
+namespace {
+// "option: 1, 2 3" inside a "; "-separated parameter string -> {1, 2, 3}; {} when absent or malformed.
+std::vector<uint64_t> param_integer_list(const std::string &option, const std::string &param) {
+    size_t at = 0;
+    while ((at = param.find(option, at)) != std::string::npos) {
+        const bool starts = at == 0 || param[at - 1] == ' ' || param[at - 1] == ';' || param[at - 1] == '\t';
+        size_t p = at + option.size();
+        while (p < param.size() && (param[p] == ' ' || param[p] == '\t')) ++p;
+        if (starts && p < param.size() && param[p] == ':') {
+            std::vector<uint64_t> ret;
+            ++p;
+            while (p < param.size() && param[p] != ';') {
+                const char c = param[p];
+                if (c == ' ' || c == ',' || c == '\t') {
+                    ++p;
+                } else if (c >= '0' && c <= '9') {
+                    uint64_t v = 0;
+                    while (p < param.size() && param[p] >= '0' && param[p] <= '9') v = v * 10 + (param[p++] - '0');
+                    ret.push_back(v);
+                } else {
+                    return {};
+                }
+            }
+            return ret;
+        }
+        at += option.size();
+    }
+    return {};
+}
+} // namespace
+
+std::string EngineHip::userKernel(const std::string &kernel, std::vector<bh_view> &operands, const std::string &param) {
+    for (const bh_view &v : operands)
+        if (v.isConstant()) return "[HIP] userKernel: operands cannot be constants";
+    const std::vector<uint64_t> gws = param_integer_list("global_work_size", param);
+    const std::vector<uint64_t> lws = param_integer_list("local_work_size", param);
+    if (gws.size() != lws.size())
+        return "[HIP] userKernel-param dimension of global_work_size and local_work_size must be the same";
+    if (gws.empty() || gws.size() > 3)
+        return "[HIP] userKernel-param needs global_work_size and local_work_size of one to three dimensions";
+    uint32_t grid[3] = {1, 1, 1}, block[3] = {1, 1, 1};
+    uint64_t threads = 1;
+    for (size_t d = 0; d < gws.size(); ++d) {
+        if (lws[d] == 0 || lws[d] > 1024) return "[HIP] userKernel-param local_work_size must be in 1..1024";
+        const uint64_t g = (gws[d] + lws[d] - 1) / lws[d];
+        if (g > 0x7fffffffull) return "[HIP] userKernel-param global_work_size too large";
+        grid[d] = static_cast<uint32_t>(g);
+        block[d] = static_cast<uint32_t>(lws[d]);
+        threads *= lws[d];
+    }
+    if (threads > 1024) return "[HIP] userKernel-param local_work_size has more than 1024 threads";
+    const auto t = std::chrono::steady_clock::now();
+    if (dry_run) {
+        if (bhhip_compile("", kernel.c_str(), "execute", cache_dir.c_str(), nullptr) != 0)
+            return std::string("[HIP] ") + bhhip_last_error();
+        stats.time_compile += seconds_since(t);
+        return "";
+    }
+    void *fn = nullptr;
+    if (bhhip_compile("", kernel.c_str(), "execute", cache_dir.c_str(), &fn) != 0)
+        return std::string("[HIP] ") + bhhip_last_error();
+    stats.time_compile += seconds_since(t);
+    std::vector<void *> ptrs;
+    for (const bh_view &v : operands) {
+        ensureResident(v.base);
+        ptrs.push_back(bufs.at(v.base).ptr);
+    }
+    void *none = nullptr;
+    if (bhhip_launch_3d(fn, grid, block, ptrs.empty() ? static_cast<const void *>(&none) : ptrs.data(),
+                        ptrs.size() * sizeof(void *)) != 0)
+        return std::string("[HIP] ") + bhhip_last_error();
+    ++stats.launches;
+    ++stats.launches_by_strategy["user"];
+    // The kernel may write any operand.
+    for (const bh_view &v : operands) markDeviceWritten(v.base);
+    return "";
+}
+
+void EngineHip::markHostWritten(bh_base *base) {
+    auto it = bufs.find(base);
+    if (it == bufs.end() || base->getDataPtr() == nullptr) return;
+    it->second.host_valid = true;
+    it->second.dev_valid = false;
+}
+
 void EngineHip::markDeviceWritten(bh_base *base) {
     DeviceBuffer &d = buffer(base);
     d.dev_valid = true;
@@ -947,9 +1032,12 @@ public:
         if (tag != "hip") {
             if (!child.initiated()) throw std::runtime_error("No backend with tag \"" + tag + "\" found");
             for (const bh_view &v : operand_list) engine.copyToHost(v.base);
-            return child.userKernel(kernel, operand_list, compile_cmd, tag, param);
+            const std::string ret = child.userKernel(kernel, operand_list, compile_cmd, tag, param);
+            // The child may have written any operand's host data.
+            for (const bh_view &v : operand_list) engine.markHostWritten(v.base);
+            return ret;
         }
-        throw std::runtime_error("hip engine: userKernel is not implemented");
+        return engine.userKernel(kernel, operand_list, param);
     }
 };
 

@@ -67,7 +67,14 @@ public:
     void setMemoryPointer(bh_base *base, bool host_ptr, void *mem);
     /* The device buffer of `base` now holds its newest data (written outside a kernel launch). */
     void markDeviceWritten(bh_base *base);
+    /* The host data of `base` now holds its newest data (written by a child component). */
+    void markHostWritten(bh_base *base);
     void copyAllToHost();
+    /* Compile the user's `extern "C" __global__ void execute(T0 *a0, ...)` with hiprtc and launch it on the
+     * operands' device buffers with the shape in `param` ("global_work_size: X [Y [Z]]; local_work_size: ...").
+     * Returns "" or an error message, like the reference's OpenCL engine
+     * (ve/opencl/engine_opencl.cpp:611-669). */
+    std::string userKernel(const std::string &kernel, std::vector<bh_view> &operands, const std::string &param);
     void copyToHost(bh_base *base);
     std::string info() const;
     std::string statistics() const;

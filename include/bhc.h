@@ -15,6 +15,8 @@
  *   bhc_data_set       <- bhc_data_set                     (setMemoryPointer, bh_component.hpp:201-211)
  *   bhc_message        <- bhc_message                      (message, bh_component.hpp:180-186)
  *   bhc_extmethod      <- bhc_extmethod                    (Runtime::enqueueExtmethod, Runtime.hpp:243-261)
+ *   bhc_user_kernel    <- bhc_user_kernel                  (bridge/c/gen_specials.py:286-304,
+ *                                                          Runtime::userKernel, bridge/cxx/src/Runtime.cpp:131-138)
  *
  * Errors: functions return 0 (or a non-null handle) on success; on failure
  * they return non-zero (or NULL) and bhc_last_error() holds the message of the
@@ -75,6 +77,12 @@ int bhc_data_copy_out(bhc_base_t base, void *dst, int64_t nbytes);
 const char *bhc_message(const char *msg);
 /* Register extension method `name` and queue one instruction of it. */
 int bhc_extmethod(const char *name, int nops, const bhc_view_t *views);
+/* Compile and run a user kernel on the operands (`tag` selects the component, "hip" for this engine).
+ * Returns "" on success, else the error message; the string stays valid until the next call.  Returns
+ * NULL only when the call itself failed (bhc_last_error()).  The caller flushes first, as the reference
+ * front ends do (bridge/bh107/bh107/user_kernel.py:88-90). */
+const char *bhc_user_kernel(const char *kernel, int nops, const bhc_view_t *operands, const char *compile_cmd,
+                            const char *tag, const char *param);
 /* Auto-flush after this many queued instructions (default 1000 like the reference; <= 0 disables). */
 int bhc_set_flush_threshold(int64_t n);
 int64_t bhc_base_nelem(bhc_base_t base);

@@ -69,6 +69,10 @@ void bhhip_compile_stats(uint64_t *compiles, uint64_t *memory_hits, uint64_t *di
 /* Launch `function` on the engine stream with a packed argument buffer
  * (each argument at its natural alignment). */
 int bhhip_launch(void *function, uint64_t grid, uint32_t block, const void *args, size_t args_bytes);
+/* The same with a 3-D grid (workgroup counts per dimension) and a 3-D workgroup shape; used for user
+ * kernels, whose launch shape the caller chooses (reference ve/opencl/engine_opencl.cpp:632-662). */
+int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block[3], const void *args,
+                    size_t args_bytes);
 int bhhip_synchronize(void);
 
 /* Timing: an event pair around stream work. */
