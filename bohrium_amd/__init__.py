@@ -15,6 +15,8 @@ from .bharray import (BhArray, array, empty, full, zeros, ones, arange, take, pu
                       assign, asnumpy_many, Iterator, get_iterator)
 from . import random
 from . import user_kernel
+from . import blas
+from .blas import matmul
 
 _GLOBAL_UFUNCS = ["add", "subtract", "multiply", "true_divide", "divide", "floor_divide", "power", "absolute",
                   "greater", "greater_equal", "less", "less_equal", "equal", "not_equal", "logical_and",

@@ -1,0 +1,79 @@
+"""Basic Linear Algebra Subprograms on the device (the blas_* extension methods).
+
+Mirrors the reference's bohrium.blas (bridge/npbackend/bohrium/blas.py:12-117): operands are made
+C-contiguous, alpha scales A and beta scales C in bytecode, and one extension-method instruction
+(C, A[, B]) is queued; the hip engine runs it from libbh_blas_hip.so on the matrix cores.  As in the
+reference, the extension method itself computes with alpha 1 and beta 0, so `beta` only affects the
+C that is passed in and then overwritten (extmethods/blas/templates/methods.json "options").
+"""
+import sys
+
+import numpy as np
+
+from . import _bh_api
+from .bharray import BhArray, empty
+
+_TYPES = (np.float32, np.float64, np.complex64, np.complex128)
+
+
+def __blas(name, a, b, alpha=1.0, c=None, beta=0.0, shape_matters=True, c_shape=None):
+    if b is not None:
+        if not (a.ndim == 2 and b.ndim == 2):
+            sys.stderr.write("[ext] Matrices need to be two-dimensional.\n")
+            return None
+        if a.shape[1] != b.shape[0] and shape_matters:
+            sys.stderr.write("[ext] Wrong shape of matrices: first argument has shape {} and second has shape {}.\n"
+                             .format(a.shape, b.shape))
+            return None
+        if not b.iscontiguous():
+            b = b.copy()
+    if not a.iscontiguous():
+        a = a.copy()
+    if np.dtype(a.dtype).type not in _TYPES:
+        raise TypeError("%s not supported by BLAS for '%s'" % (a.dtype, name))
+    if c is None:
+        c = empty(c_shape or (a.shape[0], b.shape[1]), a.dtype)
+    elif not c.iscontiguous():
+        c = c.copy()
+    if alpha != 1.0:
+        a = a * alpha
+    if beta != 0.0:
+        c = c * beta
+    ops = [c.view_desc(), a.view_desc()] + ([b.view_desc()] if b is not None else [])
+    _bh_api.extmethod(name, ops)  # writes c
+    return c
+
+
+def gemm(a, b, alpha=1.0, c=None, beta=0.0):
+    """C := alpha * A * B"""
+    return __blas("blas_gemm", a, b, alpha, c, beta)
+
+
+def gemmt(a, b, alpha=1.0, c=None, beta=0.0):
+    """C := alpha * A^T * B  (A m x k, B m x n, C k x n)"""
+    if a.ndim == 2 and b.ndim == 2 and a.shape[0] != b.shape[0]:
+        sys.stderr.write("[ext] Wrong shape of matrices: first argument has shape {} and second has shape {}.\n"
+                         .format(a.shape, b.shape))
+        return None
+    return __blas("blas_gemmt", a, b, alpha, c, beta, shape_matters=False,
+                  c_shape=(a.shape[1], b.shape[1]) if a.ndim == 2 and b.ndim == 2 else None)
+
+
+def syrk(a, alpha=1.0, c=None, beta=0.0):
+    """C := alpha * A * A**T (upper triangle of C; its strictly lower part keeps the values of the C given)"""
+    if a.ndim != 2:
+        sys.stderr.write("[ext] Matrices need to be two-dimensional.\n")
+        return None
+    return __blas("blas_syrk", a, None, alpha, c, beta, c_shape=(a.shape[0], a.shape[0]))
+
+
+def matmul(a, b):
+    """2-d matrix product on the matrix cores for float and complex operands."""
+    if not (isinstance(a, BhArray) and isinstance(b, BhArray)) or a.ndim != 2 or b.ndim != 2:
+        raise ValueError("matmul: two 2-d arrays expected")
+    if a.shape[1] != b.shape[0]:
+        raise ValueError("matmul: shapes %s and %s do not align" % (a.shape, b.shape))
+    if a.dtype != b.dtype:
+        t = np.promote_types(a.dtype, b.dtype)
+        a, b = a.astype(t), b.astype(t)
+    return gemm(a, b)

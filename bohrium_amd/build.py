@@ -4,6 +4,7 @@
     libbhc.so           the bhc C-ABI bridge runtime (g++)
     libbh_vem_node.so   the pass-through node manager component (g++)
     libbh_ve_hip.so     the hip vector engine component + bhhip C-ABI (hipcc, gfx950)
+    libbh_blas_hip.so   blas_* extension methods on the matrix cores (hipcc, gfx950)
 
 Every call recompiles; objects go to bohrium_amd/_build/.
 """
@@ -32,6 +33,7 @@ BRIDGE = ["bridge/runtime.cpp"]
 NODE = ["vem_node/node.cpp"]
 VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp", "ve_hip/features.cpp"]
 VE_HIP_HIP = ["ve_hip/bhhip.cpp", "ve_hip/engine.cpp", "ve_hip/batch.cpp", "ve_hip/costmodel.hip"]
+BLAS_HIP = ["ext_blas/blas_hip.hip"]
 
 
 def _run(cmd):
@@ -47,7 +49,8 @@ def _obj(src):
 
 def _compile(src, hip=False):
     out = _obj(src)
-    cmd = ([HIPCC] + HIPFLAGS if hip else ["g++"] + CXXFLAGS) + ["-c", os.path.join(CSRC, src), "-o", out]
+    flags = HIPFLAGS + (["-O3"] if src in BLAS_HIP else [])
+    cmd = ([HIPCC] + flags if hip else ["g++"] + CXXFLAGS) + ["-c", os.path.join(CSRC, src), "-o", out]
     _run(cmd)
     return out
 
@@ -55,7 +58,7 @@ def _compile(src, hip=False):
 def build(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
-    jobs = [(s, False) for s in CORE + BRIDGE + NODE + VE_HIP_CXX] + [(s, True) for s in VE_HIP_HIP]
+    jobs = [(s, False) for s in CORE + BRIDGE + NODE + VE_HIP_CXX] + [(s, True) for s in VE_HIP_HIP + BLAS_HIP]
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         objs = dict(zip([s for s, _ in jobs], ex.map(lambda j: _compile(*j), jobs)))
     rpath = "-Wl,-rpath,$ORIGIN"
@@ -68,6 +71,9 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "-o", os.path.join(LIB, "libbh_ve_hip.so")]
          + [objs[s] for s in VE_HIP_CXX + VE_HIP_HIP]
          + ["-L" + LIB, "-lbh", rpath, "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64",
+            "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    _run([HIPCC, "-shared", "-fPIC", "-o", os.path.join(LIB, "libbh_blas_hip.so")] + [objs[s] for s in BLAS_HIP]
+         + ["-L" + LIB, "-lbh", rpath, "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
             "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     # CPython extension with the per-operation bridge calls.
     import sysconfig

@@ -282,6 +282,8 @@ int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block
     return 0;
 }
 
+void *bhhip_stream(void) { return g_stream; }
+
 int bhhip_synchronize(void) {
     HIPCHK(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
     return 0;

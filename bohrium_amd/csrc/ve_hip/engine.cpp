@@ -11,6 +11,7 @@
 
 #include <bohrium/bh_main_memory.hpp>
 #include "../../../include/bhhip.h"
+#include "../../../include/bh_ve_hip.h"
 
 namespace bohrium {
 namespace vehip {
@@ -608,7 +609,17 @@ void EngineHip::handleExtmethod(BhIR *bhir) {
             instr_list.clear();
             for (const bh_view &v : instr.operand)
                 if (!v.isConstant()) ensureResident(v.base);
-            ext->second.execute(&instr, this);
+            bh_hip_ext_context ctx;
+            ctx.version = 1;
+            ctx.dry_run = dry_run ? 1 : 0;
+            ctx.engine = this;
+            ctx.device_pointer = [](void *engine, const void *base) -> void * {
+                EngineHip *e = static_cast<EngineHip *>(engine);
+                bh_base *b = const_cast<bh_base *>(static_cast<const bh_base *>(base));
+                return e->getMemoryPointer(*b, false, false, false);
+            };
+            ctx.stream = dry_run ? nullptr : bhhip_stream();
+            ext->second.execute(&instr, &ctx);
             if (!instr.operand.empty() && !instr.operand[0].isConstant()) {
                 DeviceBuffer &d = buffer(instr.operand[0].base);
                 d.dev_valid = true;
@@ -907,7 +918,11 @@ public:
 
     void extmethod(const std::string &name, bh_opcode opcode) override {
         try {
-            extmethods.insert(std::make_pair(opcode, extmethod::ExtmethodFace(config.defaultGetList("libs", {}), name)));
+            // Library paths with a directory part are relative to the config file, like `impl`.
+            std::vector<std::string> libs = config.defaultGetList("libs", {});
+            for (std::string &l : libs)
+                if (!l.empty() && l[0] != '/' && l.find('/') != std::string::npos) l = config.file_dir + "/" + l;
+            extmethods.insert(std::make_pair(opcode, extmethod::ExtmethodFace(libs, name)));
         } catch (const std::out_of_range &) {
             if (!child.initiated()) throw;
             child.extmethod(name, opcode);

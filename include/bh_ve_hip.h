@@ -29,6 +29,19 @@ extern "C" {
 typedef struct bh_component_impl bh_component_impl;
 #endif
 
+/* What the engine hands an extension method as the `arg` of ExtmethodImpl::execute(instr, arg).
+ * The reference GPU engines pass the engine object itself (core engine_gpu.hpp:177); this engine
+ * passes a plain C view of what a device method needs.  Every operand of the instruction is
+ * resident on the device when execute() is called, and operand 0 counts as written afterwards. */
+typedef struct bh_hip_ext_context {
+    int version;     /* 1 */
+    int dry_run;     /* compile-only engine: check the operands, launch nothing */
+    void *engine;
+    /* Device address of a bh_base's buffer (element 0 of the base). */
+    void *(*device_pointer)(void *engine, const void *base);
+    void *stream;    /* hipStream_t every engine launch is ordered on */
+} bh_hip_ext_context;
+
 /* Create the engine at `stack_level` (reads its [hip] config section). */
 bh_component_impl *create(int stack_level);
 /* Destroy an engine created by create(); device memory is released. */
