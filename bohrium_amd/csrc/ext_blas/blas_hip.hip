@@ -7,9 +7,9 @@
  * rows padded by 16 elements so that the 16-lane groups of one MFMA operand
  * read (lanes 0-15: row k, lanes 16-31: row k + 1, ...) fall in disjoint banks.
  * The next K step is fetched into registers while the current one runs on the
- * matrix cores.  MFMA operand layout (16 x 16 x 4): lane l supplies
- * A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[4 (l / 16) + r][l % 16],
- * r = 0..3.
+ * matrix cores.  MFMA operand layout (16 x 16 x 4, measured on gfx950): lane l
+ * supplies A[l % 16][l / 16] and B[l / 16][l % 16]; its 4 results are
+ * D[4 (l / 16) + r][l % 16] for f32 and D[4 r + l / 16][l % 16] for f64, r = 0..3.
  *
  * Complex types use a 16 x 16 LDS-tiled kernel with complex fused multiply-adds.
  */
@@ -36,12 +36,15 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ Acc step(double a, double b, Acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // row of the 16 x 16 tile that result r of `lane` belongs to
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 template <> struct Mfma<float> {
     typedef bh_f4 Acc;
     static __device__ __forceinline__ Acc step(float a, float b, Acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int kThreads = 256;
@@ -138,7 +141,7 @@ gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, i
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t row = bm + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int64_t row = bm + wm + i * 16 + Mfma<T>::row(lane, r);
                 const int64_t col = bn + wn + j * 16 + (lane & 15);
                 if (row < M && col < N && (!UPPER || row <= col)) C[row * ldc + col] = acc[i][j][r];
             }

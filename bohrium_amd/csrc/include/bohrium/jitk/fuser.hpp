@@ -37,6 +37,9 @@ struct FusionConfig {
     /* Keep an accumulate out of the pre-fuser's groups that hold any other sweep (a GPU scan
      * skeleton cannot also reduce); the greedy fuser applies merge_filter instead. */
     bool accumulate_alone = false;
+    /* Keep sweeps along different axes of one shape in different pre-fuser groups (a loop nest can
+     * make only one of them its inner reduction). */
+    bool one_sweep_axis = false;
     bool can_merge(const Block &a, const Block &b) const {
         if (!mergeable(a, b, avoid_rank0_sweep)) return false;
         return !merge_filter || a.isInstr() || b.isInstr() || merge_filter(a.getLoop(), b.getLoop());
@@ -45,7 +48,8 @@ struct FusionConfig {
 
 std::vector<InstrPtr> simplify_instr_list(const std::vector<bh_instruction *> &instr_list);
 void simplify_instr(bh_instruction &instr);
-std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone = false);
+std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone = false,
+                                   bool one_sweep_axis = false);
 std::vector<Block> fuser_singleton(const std::vector<bh_instruction *> &instr_list);
 void fuser_serial(std::vector<Block> &block_list, const FusionConfig &config);
 void fuser_greedy(const FusionConfig &config, std::vector<Block> &block_list, const MergeWeight &weight_fn);

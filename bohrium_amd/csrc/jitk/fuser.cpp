@@ -110,7 +110,8 @@ std::vector<InstrPtr> simplify_instr_list(const std::vector<bh_instruction *> &i
     return ret;
 }
 
-std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone) {
+std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone,
+                                   bool one_sweep_axis) {
     const std::vector<InstrPtr> list = simplify_instr_list(instr_list);
     std::vector<std::vector<InstrPtr> > groups;
     for (auto it = list.begin(); it != list.end();) {
@@ -123,10 +124,13 @@ std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_li
         }
         bool has_acc = bh_opcode_is_accumulate((*it)->opcode) != 0;
         bool has_sweep = bh_opcode_is_sweep((*it)->opcode) != 0;
+        int axis = has_sweep ? (*it)->sweep_axis() : -1;
         ++it;
         for (; it != list.end(); ++it) {
             const bool acc = bh_opcode_is_accumulate((*it)->opcode) != 0;
             const bool sweep = bh_opcode_is_sweep((*it)->opcode) != 0;
+            // Sweeps along two axes of one shape cannot both be inner reductions of a loop nest.
+            if (one_sweep_axis && sweep && axis >= 0 && (*it)->sweep_axis() != axis) break;
             if (accumulate_alone && ((has_acc && sweep) || (acc && has_sweep))) {
                 if (acc) {
                     // The element-wise instructions at the end of the group that only feed this
@@ -166,6 +170,7 @@ std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_li
                 g.push_back(*it);
                 has_acc |= acc;
                 has_sweep |= sweep;
+                if (sweep && axis < 0) axis = (*it)->sweep_axis();
             } else {
                 break;
             }
@@ -696,7 +701,7 @@ std::vector<Block> get_block_list(const std::vector<bh_instruction *> &instr_lis
     if (config.pre_fuser == "none" || config.pre_fuser == "singleton") {
         block_list = fuser_singleton(instr_list);
     } else if (config.pre_fuser == "lossy" || config.pre_fuser == "pre_fuser_lossy") {
-        block_list = pre_fuser_lossy(instr_list, config.accumulate_alone);
+        block_list = pre_fuser_lossy(instr_list, config.accumulate_alone, config.one_sweep_axis);
     } else {
         throw std::runtime_error("Unknown pre-fuser: " + config.pre_fuser);
     }

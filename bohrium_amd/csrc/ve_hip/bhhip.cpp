@@ -284,6 +284,15 @@ int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block
 
 void *bhhip_stream(void) { return g_stream; }
 
+int bhhip_occupancy(void *function, uint32_t block, int *blocks_per_cu) {
+    int n = 0;
+    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<hipFunction_t>(function),
+                                                              static_cast<int>(block), 0),
+           "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    *blocks_per_cu = n;
+    return 0;
+}
+
 int bhhip_synchronize(void) {
     HIPCHK(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
     return 0;

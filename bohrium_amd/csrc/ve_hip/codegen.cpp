@@ -1,5 +1,7 @@
 #include "codegen.hpp"
 
+#include <bohrium/jitk/fuser.hpp>
+
 #include <algorithm>
 #include <functional>
 #include <cstring>
@@ -1292,10 +1294,8 @@ KernelPlan Writer::plan(Strategy s) {
     if (s == Strategy::FULL_REDUCE) {
         // The finalize kernel folds this many partials (the main kernel's grid).
         params.push_back({"uint32_t", "bh_nparts", false});
-        const uint32_t n = static_cast<uint32_t>(kp.grid);
         ArgSpec a;
-        a.kind = ArgSpec::LITERAL;
-        a.bytes.assign(reinterpret_cast<const uint8_t *>(&n), reinterpret_cast<const uint8_t *>(&n) + 4);
+        a.kind = ArgSpec::GRID_U32;
         a.size = 4;
         a.align = 4;
         kp.args.push_back(a);
@@ -1367,7 +1367,24 @@ Strategy classify_root_block(const LoopB &l0) {
     return Strategy::SERIAL;
 }
 
+namespace {
+// Sweep axis per input rank of every sweep in `l` (nested loops included).
+void sweep_axes(const LoopB &l, std::map<int64_t, std::set<int> > &out) {
+    for (const InstrPtr &i : jitk::allInstr(l))
+        if (bh_opcode_is_sweep(i->opcode) && i->operand.size() > 1 && !i->operand[1].isConstant())
+            out[i->operand[1].ndim].insert(i->sweep_axis());
+}
+} // namespace
+
 bool root_merge_allowed(const LoopB &l1, const LoopB &l2) {
+    // Sweeps along two axes of inputs of one rank (a column sum and a row sum of one shape) cannot
+    // both be inner reductions of one loop nest, at any depth of the nest; the merged block would
+    // run on one thread.  (A full reduction through a row temporary reduces inputs of two ranks.)
+    std::map<int64_t, std::set<int> > axes;
+    sweep_axes(l1, axes);
+    sweep_axes(l2, axes);
+    for (const auto &kv : axes)
+        if (kv.first > 1 && kv.second.size() > 1) return false;
     if (l1.rank != 0 || l2.rank != 0) return true;
     size_t nsweeps = l1._sweeps.size() + l2._sweeps.size();
     bool acc = false;

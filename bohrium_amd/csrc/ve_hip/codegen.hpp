@@ -34,7 +34,8 @@ const char *strategy_name(Strategy s);
 /* One kernel argument: either literal bytes known at plan time, or a pointer
  * the engine resolves at launch time. */
 struct ArgSpec {
-    enum Kind { LITERAL, BASE_PTR, SCRATCH_PTR, COUNTER_PTR, ERRFLAG_PTR } kind = LITERAL;
+    /* GRID_U32: the launch's workgroup count as uint32 (the finalize kernel's partial count). */
+    enum Kind { LITERAL, BASE_PTR, SCRATCH_PTR, COUNTER_PTR, ERRFLAG_PTR, GRID_U32 } kind = LITERAL;
     const bh_base *base = nullptr;
     int64_t scratch_offset = 0;  // byte offset into the engine's scratch area
     std::vector<uint8_t> bytes;  // LITERAL value
@@ -96,6 +97,9 @@ struct WriterConfig {
     bool row_reduce = true;  // run row reductions with one wave per row (ROW_REDUCE)
     int waves_per_eu = 0;    // >0: amdgpu_waves_per_eu occupancy hint on the main kernels
     int reduce_unroll = 1;   // the same for FULL_REDUCE
+    /* Grid-stride skeletons (MAP, FULL_REDUCE, ROW_REDUCE) launch at most the workgroups that are
+     * resident on the device at once (CUs x occupancy of the compiled kernel). */
+    bool resident_grid = true;
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
 

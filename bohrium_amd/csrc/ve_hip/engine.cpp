@@ -107,6 +107,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
         check(bhhip_set_arch(cfg.defaultGet<std::string>("dry_run_arch", "gfx950").c_str()), "set arch");
     } else {
         check(bhhip_init(cfg.defaultGet<int>("device_number", 0)), "init");
+        check(bhhip_device_props(&num_cus, nullptr), "device properties");
     }
     fusion_config.pre_fuser = cfg.defaultGet<std::string>("pre_fuser", "lossy");
     fusion_config.fuser_list =
@@ -115,11 +116,13 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     fusion_config.avoid_rank0_sweep = false;
     fusion_config.merge_filter = [](const jitk::LoopB &a, const jitk::LoopB &b) { return root_merge_allowed(a, b); };
     fusion_config.accumulate_alone = true;
+    fusion_config.one_sweep_axis = true;
     fusion_config.learned_cost = [](const std::vector<std::vector<jitk::InstrPtr> > &sets, std::vector<double> &us) {
         return predict_block_times(sets, us);
     };
     wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
     wcfg.reduce_max_grid = cfg.defaultGet<int>("reduce_max_grid", 4096);
+    wcfg.resident_grid = cfg.defaultGet<bool>("resident_grid", true);
     wcfg.reduce_min_items = std::max(1, cfg.defaultGet<int>("reduce_min_items", 4));
     wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
     wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
@@ -310,6 +313,9 @@ void EngineHip::dumpLaunch(const char *dir, const KernelPlan &plan, const std::v
         if (a.kind == ArgSpec::BASE_PTR) {
             const bh_base *b = ord2base[a.base_ordinal];
             std::fprintf(f, "base %lld %d\n", static_cast<long long>(b->nelem()), bh_type_size(b->dtype()));
+        } else if (a.kind == ArgSpec::GRID_U32) {
+            const uint32_t g = static_cast<uint32_t>(plan.grid);
+            std::fprintf(f, "bytes %u %u %u %u\n", g & 255u, (g >> 8) & 255u, (g >> 16) & 255u, g >> 24);
         } else if (a.kind != ArgSpec::LITERAL) {
             std::fprintf(f, "engine\n");
         } else {
@@ -341,6 +347,16 @@ void EngineHip::compilePlan(KernelPlan &plan) {
     if (!plan.fin_name.empty())
         check(bhhip_compile(kPrelude, plan.source.c_str(), plan.fin_name.c_str(), cache_dir.c_str(), &plan.fin_function),
               "kernel compilation");
+    // A grid-stride launch with more workgroups than fit on the device at once runs a second, partial
+    // round on a fraction of the CUs; one resident round of workgroups does the same work without it.
+    const bool grid_stride = plan.strategy == Strategy::MAP || plan.strategy == Strategy::FULL_REDUCE ||
+                             plan.strategy == Strategy::ROW_REDUCE;
+    if (wcfg.resident_grid && grid_stride && num_cus > 0) {
+        int per_cu = 0;
+        check(bhhip_occupancy(plan.function, plan.block, &per_cu), "occupancy query");
+        const uint64_t cap = static_cast<uint64_t>(num_cus) * static_cast<uint64_t>(std::max(1, per_cu));
+        if (plan.grid > cap) plan.grid = cap;
+    }
 }
 
 size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
@@ -376,6 +392,11 @@ size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> 
                 ptr = dry_run ? reinterpret_cast<void *>(static_cast<uintptr_t>(a.base_ordinal + 1) << 32)
                               : bufs.at(ord2base[a.base_ordinal]).ptr;
                 break;
+            case ArgSpec::GRID_U32: {
+                const uint32_t g = static_cast<uint32_t>(plan.grid);
+                put(&g, 4, 4);
+                continue;
+            }
             case ArgSpec::SCRATCH_PTR: ptr = scratch; break;
             case ArgSpec::COUNTER_PTR: ptr = counters; break;
             case ArgSpec::ERRFLAG_PTR: ptr = counters + 1; break;
@@ -959,7 +980,7 @@ public:
                 if (eq == std::string::npos) throw std::invalid_argument("hip engine: bad codegen setting: " + kv);
                 const std::string k = kv.substr(0, eq);
                 const int v = std::stoi(kv.substr(eq + 1));
-                if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu"))
+                if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu" && k != "resident_grid"))
                     throw std::invalid_argument("hip engine: codegen setting must be >= 1: " + kv);
                 if (k == "max_grid") {
                     engine.codegenConfig().max_grid = v;
@@ -967,6 +988,8 @@ public:
                     engine.codegenConfig().map_unroll = v;
                 } else if (k == "reduce_unroll") {
                     engine.codegenConfig().reduce_unroll = v;
+                } else if (k == "resident_grid") {
+                    engine.codegenConfig().resident_grid = v != 0;
                 } else if (k == "reduce_max_grid") {
                     engine.codegenConfig().reduce_max_grid = v;
                 } else if (k == "reduce_min_items") {
@@ -985,7 +1008,7 @@ public:
             ss << "max_grid=" << w.max_grid << " map_unroll=" << w.map_unroll << " reduce_unroll=" << w.reduce_unroll
                << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items
                << " scan_items=" << w.scan_items << " row_reduce=" << (w.row_reduce ? 1 : 0)
-               << " waves_per_eu=" << w.waves_per_eu;
+               << " waves_per_eu=" << w.waves_per_eu << " resident_grid=" << (w.resident_grid ? 1 : 0);
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));

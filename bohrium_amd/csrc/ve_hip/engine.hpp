@@ -146,6 +146,7 @@ private:
     RewriteStore rewrite_store;  // instructions and temporaries added to the current flush
     std::string cache_dir;
     bool resident = true;
+    int num_cus = 0;  // compute units of the device (0 in compile-only mode)
     std::unordered_map<const bh_base *, DeviceBuffer> bufs;
     /* Bases released during the current flush; a BH_FREE that no kernel released is caught with it. */
     std::set<const bh_base *> released_now;

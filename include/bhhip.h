@@ -74,6 +74,8 @@ int bhhip_launch(void *function, uint64_t grid, uint32_t block, const void *args
 int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block[3], const void *args,
                     size_t args_bytes);
 int bhhip_synchronize(void);
+/* Workgroups of `block` threads of `function` that can be resident on one compute unit at once. */
+int bhhip_occupancy(void *function, uint32_t block, int *blocks_per_cu);
 /* The engine stream (a hipStream_t), for device code launched outside this layer (extension methods). */
 void *bhhip_stream(void);
 

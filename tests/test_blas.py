@@ -102,11 +102,12 @@ def test_gemm_alpha_and_views():
     # a transposed view is copied to a contiguous operand first; alpha scales A
     c = bh.blas.gemm(bh.array(np.ascontiguousarray(a.T)).T, bh.array(b), alpha=2.5)
     _check(c.asnumpy() / 2.5, a, b, np.float64)
-    m = bh.matmul(A, bh.array(b.astype(np.float32)))  # promoted to float64
-    _check(m.asnumpy(), a, b, np.float64)
+    b32 = b.astype(np.float32)
+    m = bh.matmul(A, bh.array(b32))  # promoted to float64
+    _check(m.asnumpy(), a, b32.astype(np.float64), np.float64)
     # the product feeds further bytecode in the same flush sequence
     s = (bh.blas.gemm(A, bh.array(b)) * 2.0).sum().asnumpy()
-    assert abs(s - 2.0 * (a @ b).sum()) <= 1e-9 * np.abs(a) .sum() * np.abs(b).sum()
+    assert abs(s - 2.0 * (a @ b).sum()) <= 1e-9 * np.abs(a).sum() * np.abs(b).sum()
 
 
 @pytest.mark.gpu
@@ -128,9 +129,6 @@ def test_syrk_upper_triangle(dtype, m, k):
     a = _rand((m, k), dtype, 7)
     c0 = np.full((m, m), -7.0).astype(dtype)
     got = bh.blas.syrk(bh.array(a), c=bh.array(c0)).asnumpy()
-    iu = np.triu_indices(m)
-    full = np.zeros((m, m), dtype)
-    full[iu] = got[iu]
     ref_mask = np.triu(np.ones((m, m), bool))
     a64 = a.astype(np.complex128 if np.issubdtype(dtype, np.complexfloating) else np.float64)
     ref = a64 @ a64.T

@@ -39,6 +39,15 @@ plan("row_sum", lambda: bh.ones((100, 50)).sum(axis=1))
 plan("col_sum", lambda: bh.ones((100, 50)).sum(axis=0))
 plan("cumsum_axis0", lambda: bh.ones((100, 50)).cumsum(axis=0))
 plan("gather", lambda: bh.take(bh.arange(50).astype(np.float32), bh.array(np.arange(10) * 3)))
+# a random program whose column and row sums of one shape the fusers once put in one loop nest
+from bohrium_amd import _bh_api
+_bh_api.set_flush_threshold(10 ** 7)
+for fl in ("greedy, ", "serial, "):
+    bh.message("fuser_list: " + fl + "push_reductions_inwards, collapse_redundant_axes")
+    outs = workloads.random_dag(1001, 337, sides=(128, 256))
+    bh.asnumpy_many([o[:1] if o.ndim == 1 else o[:1, :1] for o in outs])
+    del outs
+    plans["dag_" + fl.split(",")[0]] = [l.split()[0] for l in bh.message("plan log").splitlines() if l.strip()]
 print("PLANS=" + json.dumps(plans))
 """
 
@@ -90,3 +99,10 @@ def test_axis_reductions_run_on_parallel_skeletons(plans):
 
 def test_gather_plan(plans):
     assert "serial" not in plans["gather"]
+
+
+def test_fusers_keep_random_programs_parallel(plans):
+    """Sweeps along two axes of one shape are never fused into one loop nest (it would run on one thread)."""
+    for name in ("dag_greedy", "dag_serial"):
+        assert plans[name], name
+        assert "serial" not in plans[name], (name, plans[name])
