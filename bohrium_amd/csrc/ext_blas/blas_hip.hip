@@ -51,22 +51,96 @@ constexpr int kThreads = 256;
 constexpr int kBK = 16;
 constexpr int kPad = 16;
 
+template <typename T> struct Vec;
+template <> struct Vec<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int n = 2;
+};
+template <> struct Vec<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int n = 4;
+};
+
+// One operand tile: ROWS rows (m of op(A) or n of op(B)) by kBK steps of k.  KCONT: k is the
+// contiguous dimension in memory (element (r, k) at P[r * ld + k]); otherwise r is
+// (element at P[k * ld + r]).  A thread moves chunks of V elements along the contiguous dimension
+// (one 16-byte load on interior tiles).  LDS layout: [ROWS][kBK + KPAD] when KCONT, else
+// [kBK][ROWS + kPad]; with these pads the 16-lane groups of an MFMA operand read (16 rows at one k,
+// the next group at k + 1) and the chunk stores fall in disjoint banks.
+template <typename T, bool KCONT, int ROWS>
+struct Tile {
+    static constexpr int V = Vec<T>::n;
+    static constexpr int PER = ROWS * kBK / kThreads;
+    static constexpr int CH = PER / V;
+    static constexpr int KPAD = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int LDS = KCONT ? ROWS * (kBK + KPAD) : kBK * (ROWS + kPad);
+    static_assert(CH * V == PER && CH * kThreads * V == ROWS * kBK, "tile / thread count");
+
+    static __device__ __forceinline__ void chunk(int c, int &r, int &k) {
+        if (KCONT) {
+            r = c / (kBK / V);
+            k = (c % (kBK / V)) * V;
+        } else {
+            k = c / (ROWS / V);
+            r = (c % (ROWS / V)) * V;
+        }
+    }
+    // vec: the whole tile is in bounds and every chunk is 16-byte aligned.
+    static __device__ __forceinline__ void fetch(const T *__restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t R,
+                                                 int64_t K, bool vec, T *reg) {
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            int r, k;
+            chunk(static_cast<int>(threadIdx.x) + e * kThreads, r, k);
+            const int64_t gr = r0 + r, gk = k0 + k;
+            if (vec) {
+                const typename Vec<T>::type v =
+                    *reinterpret_cast<const typename Vec<T>::type *>(KCONT ? P + gr * ld + gk : P + gk * ld + gr);
+#pragma unroll
+                for (int j = 0; j < V; ++j) reg[e * V + j] = v[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const int64_t rr = KCONT ? gr : gr + j, kk = KCONT ? gk + j : gk;
+                    reg[e * V + j] = (rr < R && kk < K) ? P[KCONT ? rr * ld + kk : kk * ld + rr] : T(0);
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void store(T *lds, const T *reg) {
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            int r, k;
+            chunk(static_cast<int>(threadIdx.x) + e * kThreads, r, k);
+            typename Vec<T>::type v;
+#pragma unroll
+            for (int j = 0; j < V; ++j) v[j] = reg[e * V + j];
+            *reinterpret_cast<typename Vec<T>::type *>(KCONT ? lds + r * (kBK + KPAD) + k : lds + k * (ROWS + kPad) + r) = v;
+        }
+    }
+    static __device__ __forceinline__ T read(const T *lds, int r, int k) {
+        return KCONT ? lds[r * (kBK + KPAD) + k] : lds[k * (ROWS + kPad) + r];
+    }
+};
+
 // C = op(A) op(B); op(A)(m, k) = TA ? A[k * lda + m] : A[m * lda + k],
 // op(B)(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n].  UPPER: only C(m, n) with m <= n is stored.
+// vec_ok: lda and ldb are multiples of the 16-byte vector and A, B are 16-byte aligned.
 template <typename T, bool TA, bool TB, bool UPPER, int TM, int TN>
 __global__ void __launch_bounds__(kThreads)
 gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, int64_t M, int64_t N, int64_t K,
-          int64_t lda, int64_t ldb, int64_t ldc) {
+          int64_t lda, int64_t ldb, int64_t ldc, int vec_ok) {
     constexpr int BM = 32 * TM, BN = 32 * TN;
-    constexpr int A_PER = BM * kBK / kThreads, B_PER = BN * kBK / kThreads;
-    static_assert(A_PER * kThreads == BM * kBK && B_PER * kThreads == BN * kBK, "tile / thread count");
-    __shared__ T As[kBK][BM + kPad];
-    __shared__ T Bs[kBK][BN + kPad];
+    typedef Tile<T, !TA, BM> ATile;
+    typedef Tile<T, TB, BN> BTile;
+    __shared__ __attribute__((aligned(16))) T As[ATile::LDS];
+    __shared__ __attribute__((aligned(16))) T Bs[BTile::LDS];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t bm = static_cast<int64_t>(blockIdx.y) * BM, bn = static_cast<int64_t>(blockIdx.x) * BN;
     if (UPPER && bm > bn + BN - 1) return;  // the whole tile lies below the diagonal
     const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+    const bool a_in = vec_ok && bm + BM <= M, b_in = vec_ok && bn + BN <= N;
 
     typename Mfma<T>::Acc acc[TM][TN];
 #pragma unroll
@@ -76,49 +150,18 @@ gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, i
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
 
-    // Which (row, k) of the A tile / (k, col) of the B tile element e of this thread is: consecutive
-    // threads walk the dimension that is contiguous in memory.
-    auto a_idx = [&](int e, int &mm, int &kk) {
-        const int idx = t + e * kThreads;
-        if (TA) { kk = idx / BM; mm = idx % BM; } else { mm = idx / kBK; kk = idx % kBK; }
-    };
-    auto b_idx = [&](int e, int &kk, int &nn) {
-        const int idx = t + e * kThreads;
-        if (TB) { nn = idx / kBK; kk = idx % kBK; } else { kk = idx / BN; nn = idx % BN; }
-    };
-    T ra[A_PER], rb[B_PER];
+    T ra[ATile::PER], rb[BTile::PER];
     auto fetch = [&](int64_t k0) {
-#pragma unroll
-        for (int e = 0; e < A_PER; ++e) {
-            int mm, kk;
-            a_idx(e, mm, kk);
-            const int64_t gm = bm + mm, gk = k0 + kk;
-            ra[e] = (gm < M && gk < K) ? (TA ? A[gk * lda + gm] : A[gm * lda + gk]) : T(0);
-        }
-#pragma unroll
-        for (int e = 0; e < B_PER; ++e) {
-            int kk, nn;
-            b_idx(e, kk, nn);
-            const int64_t gk = k0 + kk, gn = bn + nn;
-            rb[e] = (gk < K && gn < N) ? (TB ? B[gn * ldb + gk] : B[gk * ldb + gn]) : T(0);
-        }
+        const bool kin = k0 + kBK <= K;
+        ATile::fetch(A, lda, bm, k0, M, K, a_in && kin, ra);
+        BTile::fetch(B, ldb, bn, k0, N, K, b_in && kin, rb);
     };
 
     fetch(0);
     for (int64_t k0 = 0; k0 < K; k0 += kBK) {
         __syncthreads();  // every wave is done reading the previous tile
-#pragma unroll
-        for (int e = 0; e < A_PER; ++e) {
-            int mm, kk;
-            a_idx(e, mm, kk);
-            As[kk][mm] = ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < B_PER; ++e) {
-            int kk, nn;
-            b_idx(e, kk, nn);
-            Bs[kk][nn] = rb[e];
-        }
+        ATile::store(As, ra);
+        BTile::store(Bs, rb);
         __syncthreads();
         if (k0 + kBK < K) fetch(k0 + kBK);
 #pragma unroll
@@ -126,9 +169,9 @@ gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, i
             const int kr = ks + (lane >> 4);
             T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = As[kr][wm + i * 16 + (lane & 15)];
+            for (int i = 0; i < TM; ++i) a[i] = ATile::read(As, wm + i * 16 + (lane & 15), kr);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bs[kr][wn + j * 16 + (lane & 15)];
+            for (int j = 0; j < TN; ++j) b[j] = BTile::read(Bs, wn + j * 16 + (lane & 15), kr);
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -185,16 +228,19 @@ thread_local std::string g_err;
 template <typename T, bool TA, bool TB, bool UPPER>
 void launch_real(int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
                  int64_t ldc, hipStream_t s) {
+    const int64_t v = Vec<T>::n;
+    const int vec_ok = lda % v == 0 && ldb % v == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(B) % 16 == 0;
     // 128 x 128 tiles when the matrix fills the device with them, 64 x 64 otherwise.
     const bool big = ((m + 127) / 128) * ((n + 127) / 128) >= 512;
     if (big) {
         const dim3 grid(static_cast<unsigned>((n + 127) / 128), static_cast<unsigned>((m + 127) / 128));
         gemm_mfma<T, TA, TB, UPPER, 4, 4><<<grid, kThreads, 0, s>>>(static_cast<const T *>(A), static_cast<const T *>(B),
-                                                                    static_cast<T *>(C), m, n, k, lda, ldb, ldc);
+                                                                    static_cast<T *>(C), m, n, k, lda, ldb, ldc, vec_ok);
     } else {
         const dim3 grid(static_cast<unsigned>((n + 63) / 64), static_cast<unsigned>((m + 63) / 64));
         gemm_mfma<T, TA, TB, UPPER, 2, 2><<<grid, kThreads, 0, s>>>(static_cast<const T *>(A), static_cast<const T *>(B),
-                                                                    static_cast<T *>(C), m, n, k, lda, ldb, ldc);
+                                                                    static_cast<T *>(C), m, n, k, lda, ldb, ldc, vec_ok);
     }
 }
 

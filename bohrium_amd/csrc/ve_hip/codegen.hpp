@@ -99,7 +99,7 @@ struct WriterConfig {
     int reduce_unroll = 1;   // the same for FULL_REDUCE
     /* Grid-stride skeletons (MAP, FULL_REDUCE, ROW_REDUCE) launch at most the workgroups that are
      * resident on the device at once (CUs x occupancy of the compiled kernel). */
-    bool resident_grid = true;
+    bool resident_grid = false;
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
 

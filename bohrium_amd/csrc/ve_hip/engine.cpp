@@ -122,7 +122,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     };
     wcfg.max_grid = cfg.defaultGet<int>("max_grid", 2048);
     wcfg.reduce_max_grid = cfg.defaultGet<int>("reduce_max_grid", 4096);
-    wcfg.resident_grid = cfg.defaultGet<bool>("resident_grid", true);
+    wcfg.resident_grid = cfg.defaultGet<bool>("resident_grid", false);
     wcfg.reduce_min_items = std::max(1, cfg.defaultGet<int>("reduce_min_items", 4));
     wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
     wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
