@@ -370,7 +370,8 @@ private:
     // Instructions a skeleton implements itself (collapsed reductions and their identities).
     std::set<const bh_instruction *> skip;
     // ROW_REDUCE: the inner loop whose iterations are spread over the 64 lanes of a wave.
-    const LoopB *lane_loop = nullptr;
+    std::set<const LoopB *> lane_loops;  // loops strided by the lanes of a wave (row skeleton)
+    bool fence_lanes = false;            // order the lanes' memory accesses after each lane loop
     // Instruction replaced by "store its input into `scan_sink`" in the scan skeleton.
     const bh_instruction *scan_instr = nullptr;
     std::string scan_sink;
@@ -577,8 +578,13 @@ void Writer::emit_block(const Block &b, Scope &sc, int ind) {
         return;
     }
     const LoopB &c = b.getLoop();
+    const bool lanes = lane_loops.count(&c) != 0;
+    if (lanes && fence_lanes) {
+        indent(body, ind);
+        body << "__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"workgroup\");\n";
+    }
     indent(body, ind);
-    if (&c == lane_loop) {
+    if (lanes) {
         body << "for (int64_t i" << c.rank << " = lane_; i" << c.rank << " < n" << lid.at(&c) << "; i" << c.rank
              << " += 64) {\n";
     } else {
@@ -587,6 +593,24 @@ void Writer::emit_block(const Block &b, Scope &sc, int ind) {
     emit_body(c, &sc, ind + 1);
     indent(body, ind);
     body << "}\n";
+    if (!lanes) return;
+    if (fence_lanes) {
+        // Elements written by one lane are read or overwritten by another (before and after every
+        // lane loop): the wave's memory accesses so far complete before it goes on (the lanes share
+        // one CU's L1).
+        indent(body, ind);
+        body << "__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"workgroup\");\n";
+    }
+    // Fold the lanes' partial reductions; every lane ends with the row's value.
+    for (const InstrPtr &sw : c._sweeps) {
+        const bh_view &v = sw->operand[0];
+        const std::string var = ref(sc, v, static_cast<int>(sw->sweep_axis()));
+        if (var.empty() || var[0] == 'a') throw std::logic_error("row skeleton: a reduction output is not in a register");
+        const std::string T = hip_type(v.base->dtype());
+        indent(body, ind);
+        body << var << " = bh_wave_reduce(" << var << ", [](" << T << " x, " << T << " y) -> " << T << " { return "
+             << combine_expr(sw->opcode, "x", "y") << "; });\n";
+    }
 }
 
 void Writer::emit_blocks(const LoopB &l, Scope &sc, int ind) {
@@ -794,49 +818,107 @@ void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
 // The inner loop of a row reduction: the innermost parallel loop of `l0` holds only constant
 // IDENTITY seeds of the reductions of one innermost sub-loop (and frees), and that sub-loop's
 // sweeps are all reductions over its own rank.  Null when `l0` has another form.
-const LoopB *row_inner_loop(const LoopB &l0) {
+// The loops a wave's 64 lanes stride when a block runs one wave per row: every loop directly
+// inside the innermost leading parallel loop P (at least one with 32 or more iterations), whose own
+// sweeps are reductions over their own, last axis; loops nested in them run inside one lane.  A base written by one of
+// them is accessed by the others only through the identical view (so each element stays with the
+// lane that wrote it), no reduction output is read inside its own loop or held in memory by an
+// accumulate, gather or scatter, and P's own instructions are element-wise.  Empty: another form.
+std::vector<const LoopB *> row_lane_loops(const LoopB &l0, bool *cross = nullptr) {
+    if (cross != nullptr) *cross = false;
     const uint64_t p = std::max<uint64_t>(1, jitk::parallel_ranks(l0).first);
     const std::vector<const LoopB *> chain = jitk::get_first_loop_blocks(l0);
-    if (chain.size() < p) return nullptr;
+    if (chain.size() < p) return {};
     const LoopB &P = *chain[p - 1];
-    const LoopB *C = nullptr;
-    for (const Block &b : P._block_list)
-        if (!b.isInstr()) {
-            if (C != nullptr) return nullptr;
-            C = &b.getLoop();
-        }
-    if (C == nullptr || !C->isInnermost() || C->_sweeps.empty() || C->size < 32) return nullptr;
-    std::set<const bh_base *> outs;
-    for (const InstrPtr &sw : C->_sweeps) {
-        if (!bh_opcode_is_reduction(sw->opcode) || sw->sweep_axis() != C->rank ||
-            sw->sweep_axis() != sw->operand[1].ndim - 1)
-            return nullptr;
-        outs.insert(sw->operand[0].base);
-    }
+    std::vector<const LoopB *> loops;
+    int64_t longest = 0;
     for (const Block &b : P._block_list) {
-        if (!b.isInstr()) continue;
-        const InstrPtr &i = b.getInstr();
-        if (bh_opcode_is_system(i->opcode)) continue;
-        if (i->opcode != BH_IDENTITY || !i->operand[1].isConstant() || !outs.count(i->operand[0].base)) return nullptr;
+        if (b.isInstr()) {
+            const InstrPtr &i = b.getInstr();
+            if (bh_opcode_is_system(i->opcode)) continue;
+            if (bh_opcode_is_sweep(i->opcode) || i->opcode == BH_GATHER || i->opcode == BH_SCATTER ||
+                i->opcode == BH_COND_SCATTER)
+                return {};
+            continue;
+        }
+        const LoopB &C = b.getLoop();
+        for (const InstrPtr &sw : C._sweeps)
+            if (!bh_opcode_is_reduction(sw->opcode) || sw->sweep_axis() != C.rank ||
+                sw->sweep_axis() != sw->operand[1].ndim - 1)
+                return {};
+        // (loops nested in C run in the lane that owns C's index, sweeps over their axes included)
+        for (const InstrPtr &i : jitk::allInstr(C))
+            if (i->opcode == BH_SCATTER || i->opcode == BH_COND_SCATTER) return {};
+        loops.push_back(&C);
+        longest = std::max(longest, C.size);
     }
-    // The reduced values may only be stored (no other instruction of the block reads them).
+    if (loops.empty() || longest < 32) return {};
+    // Reduction outputs stay in registers: not read inside their loop, not an array of another kind.
+    std::set<const bh_base *> red_outs;
+    for (const LoopB *C : loops) {
+        std::set<const bh_base *> outs;
+        for (const InstrPtr &sw : C->_sweeps) outs.insert(sw->operand[0].base);
+        for (const InstrPtr &i : jitk::allInstr(*C))
+            for (size_t o = 1; o < i->operand.size(); ++o)
+                if (!i->operand[o].isConstant() && outs.count(i->operand[o].base)) return {};
+        red_outs.insert(outs.begin(), outs.end());
+    }
     for (const InstrPtr &i : jitk::allInstr(l0))
-        for (size_t o = 1; o < i->operand.size(); ++o)
-            if (!i->operand[o].isConstant() && outs.count(i->operand[o].base)) return nullptr;
-    return C;
+        if (i->opcode == BH_GATHER && red_outs.count(i->operand[1].base)) return {};
+    // ... and are read only through the view they are reduced into (the register), since the array
+    // is written once, after the row.
+    std::map<const bh_base *, const bh_view *> red_view;
+    for (const LoopB *C : loops)
+        for (const InstrPtr &sw : C->_sweeps) red_view[sw->operand[0].base] = &sw->operand[0];
+    for (const InstrPtr &i : jitk::allInstr(l0))
+        for (size_t o = 0; o < i->operand.size(); ++o) {
+            const bh_view &v = i->operand[o];
+            if (v.isConstant()) continue;
+            auto r = red_view.find(v.base);
+            if (r != red_view.end() && !(v == *r->second)) return {};
+        }
+    // Element ownership: when what one loop (or the row's own instructions) writes is touched by
+    // another loop through another view, an element moves between lanes, and the lanes' memory
+    // accesses are ordered by a fence after each lane loop (`cross`).
+    auto writes = [](const std::vector<InstrPtr> &instrs, std::map<const bh_base *, const bh_view *> &w) {
+        for (const InstrPtr &i : instrs)
+            if (!i->operand.empty() && !i->operand[0].isConstant() && !bh_opcode_is_sweep(i->opcode))
+                w[i->operand[0].base] = &i->operand[0];
+    };
+    std::vector<std::vector<InstrPtr> > parts;
+    for (const LoopB *C : loops) parts.push_back(jitk::allInstr(*C));
+    parts.push_back(jitk::allLocalInstr(P));
+    for (size_t a = 0; a < parts.size(); ++a) {
+        std::map<const bh_base *, const bh_view *> written;
+        writes(parts[a], written);
+        for (size_t b = 0; b < parts.size(); ++b) {
+            if (a == b) continue;
+            for (const InstrPtr &i : parts[b])
+                for (const bh_view &v : i->operand) {
+                    if (v.isConstant()) continue;
+                    auto w = written.find(v.base);
+                    if (w != written.end() && !(v == *w->second) && cross != nullptr) *cross = true;
+                }
+        }
+    }
+    return loops;
 }
 
-// ---- ROW_REDUCE: one wave per row of the flattened parallel ranks
 void Writer::emit_row_reduce() {
     const LoopB &l0 = unit._block_list.back().getLoop();
     const uint64_t p = std::max<uint64_t>(1, jitk::parallel_ranks(l0).first);
     const std::vector<const LoopB *> chain = jitk::get_first_loop_blocks(l0);
-    const LoopB *C = row_inner_loop(l0);
-    if (C == nullptr) throw std::logic_error("row reduction: block has another form");
-    lane_loop = C;
+    bool cross = false;
+    const std::vector<const LoopB *> loops = row_lane_loops(l0, &cross);
+    if (loops.empty()) throw std::logic_error("row skeleton: block has another form");
+    fence_lanes = cross;
+    lane_loops.clear();
+    lane_loops.insert(loops.begin(), loops.end());
     uint64_t rows = 1;
     for (uint64_t r = 0; r < p; ++r) rows *= static_cast<uint64_t>(chain[r]->size);
-    kp.work_items = rows * static_cast<uint64_t>(C->size);
+    int64_t longest = 0;
+    for (const LoopB *C : loops) longest = std::max(longest, C->size);
+    kp.work_items = rows * static_cast<uint64_t>(longest);
     Scope ksc;
     declare_scope(unit, ksc, 1);
     body << "  const int64_t rows_ = 1";
@@ -858,16 +940,9 @@ void Writer::emit_row_reduce() {
     }
     Scope &ps = scopes[p - 1];
     const int ind = 2 + static_cast<int>(p);
+    // The row's own instructions run in every lane (same values); each lane loop is strided by the
+    // lanes and its reductions are folded across the wave right after it (emit_block).
     emit_blocks(*chain[p - 1], ps, ind);
-    // Fold the lanes' partial reductions; every lane ends with the row's value.
-    for (const InstrPtr &sw : C->_sweeps) {
-        const bh_view &v = sw->operand[0];
-        const std::string var = ref(ps, v, static_cast<int>(sw->sweep_axis()));
-        const std::string T = hip_type(v.base->dtype());
-        indent(body, ind);
-        body << var << " = bh_wave_reduce(" << var << ", [](" << T << " x, " << T << " y) -> " << T << " { return "
-             << combine_expr(sw->opcode, "x", "y") << "; });\n";
-    }
     indent(body, ind);
     body << "if (lane_ == 0) {\n";
     write_back(*chain[p - 1], ps, ind + 1);
@@ -1597,7 +1672,7 @@ std::vector<KernelPlan> plan_kernel(const LoopB &kernel, const WriterConfig &cfg
         }
         LoopB unit(-1, 1, scalar_ids);
         unit._block_list.push_back(b);
-        if (s == Strategy::MAP && cfg.row_reduce && row_inner_loop(l0) != nullptr) s = Strategy::ROW_REDUCE;
+        if (s == Strategy::MAP && cfg.row_reduce && !row_lane_loops(l0).empty()) s = Strategy::ROW_REDUCE;
         if (s == Strategy::FULL_REDUCE) {
             while (bi + 1 < blocks.size() && (blocks[bi + 1].isSystemOnly() || tail_block(blocks[bi + 1])))
                 unit._block_list.push_back(blocks[++bi]);

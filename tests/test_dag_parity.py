@@ -1,0 +1,56 @@
+"""Random array programs (the cost-model workload) on the hip engine against the oracle, under
+each fuser: every partition the fusers produce, and every skeleton the planner picks for it, must
+compute what the bytecode says.
+
+The programs (workloads.random_dag) mix element-wise float32 operations, row and column sums
+broadcast back, transposed, reversed and half-size views; each runs as one flush.  Tolerance:
+rtol 1e-4 plus an absolute 1e-5 of the output's largest magnitude, since a chain of up to a few
+hundred float32 operations with sums of 64-128 terms and cancellations (x - mean) reassociates
+and rounds differently on the device and the host; a wrong element, index or loop order is off by
+O(1) of the output's scale."""
+import numpy as np
+import pytest
+
+import bohrium_amd as bh
+from bohrium_amd import _bh_api as api
+from bohrium_amd import workloads
+from tests.helpers import run_both
+
+pytestmark = pytest.mark.gpu
+
+DEFAULT_FUSERS = "greedy, push_reductions_inwards, collapse_redundant_axes"
+FUSERS = {
+    "greedy": DEFAULT_FUSERS,
+    "serial": "serial, push_reductions_inwards, collapse_redundant_axes",
+    "none": "push_reductions_inwards, collapse_redundant_axes",
+}
+CASES = [(1000, 300), (1001, 337), (1002, 374), (1005, 135)]
+_ORACLE = {}
+
+
+@pytest.fixture(autouse=True)
+def one_flush_per_program():
+    api.set_flush_threshold(10 ** 7)
+    yield
+    api.set_flush_threshold(1000)
+    bh.message("fuser_list: " + DEFAULT_FUSERS)
+
+
+def _program(seed, n):
+    return lambda: workloads.random_dag(seed, n, sides=(64, 128))
+
+
+@pytest.mark.parametrize("fuser", sorted(FUSERS))
+@pytest.mark.parametrize("seed,n", CASES)
+def test_random_program(seed, n, fuser):
+    bh.message("fuser_list: " + FUSERS[fuser])
+    if (seed, n) not in _ORACLE:
+        got, want = run_both(_program(seed, n))
+        _ORACLE[(seed, n)] = want
+    else:
+        got = [a.asnumpy() for a in _program(seed, n)()]
+        want = _ORACLE[(seed, n)]
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert g.shape == w.shape and g.dtype == w.dtype
+        np.testing.assert_allclose(g, w, rtol=1e-4, atol=1e-5 * max(1.0, float(np.abs(w).max())))
