@@ -880,14 +880,36 @@ std::vector<const LoopB *> row_lane_loops(const LoopB &l0, bool *cross = nullptr
     // Element ownership: when what one loop (or the row's own instructions) writes is touched by
     // another loop through another view, an element moves between lanes, and the lanes' memory
     // accesses are ordered by a fence after each lane loop (`cross`).
-    auto writes = [](const std::vector<InstrPtr> &instrs, std::map<const bh_base *, const bh_view *> &w) {
-        for (const InstrPtr &i : instrs)
-            if (!i->operand.empty() && !i->operand[0].isConstant() && !bh_opcode_is_sweep(i->opcode))
-                w[i->operand[0].base] = &i->operand[0];
+    // Outputs written in a part, except the wave-folded reductions of a lane loop itself (their
+    // registers are stored after the row, and read only through their own view).
+    std::set<const bh_instruction *> folded;
+    for (const LoopB *C : loops)
+        for (const InstrPtr &sw : C->_sweeps) folded.insert(sw.get());
+    bool several = false;
+    auto writes = [&](const std::vector<InstrPtr> &instrs, std::map<const bh_base *, const bh_view *> &w) {
+        for (const InstrPtr &i : instrs) {
+            if (i->operand.empty() || i->operand[0].isConstant() || folded.count(i.get())) continue;
+            auto it = w.find(i->operand[0].base);
+            if (it != w.end() && !(*it->second == i->operand[0])) several = true;
+            w[i->operand[0].base] = &i->operand[0];
+        }
     };
     std::vector<std::vector<InstrPtr> > parts;
     for (const LoopB *C : loops) parts.push_back(jitk::allInstr(*C));
     parts.push_back(jitk::allLocalInstr(P));
+    // Inside one lane loop (its nested loops included) nothing orders the lanes: a base written
+    // there is touched there only through the written view.
+    for (size_t a = 0; a + 1 < parts.size(); ++a) {
+        std::map<const bh_base *, const bh_view *> written;
+        writes(parts[a], written);
+        if (several) return {};  // one base written through two views
+        for (const InstrPtr &i : parts[a])
+            for (const bh_view &v : i->operand) {
+                if (v.isConstant()) continue;
+                auto w = written.find(v.base);
+                if (w != written.end() && !(v == *w->second)) return {};
+            }
+    }
     for (size_t a = 0; a < parts.size(); ++a) {
         std::map<const bh_base *, const bh_view *> written;
         writes(parts[a], written);
