@@ -31,6 +31,10 @@ public:
     std::set<InstrPtr> _sweeps;
     std::set<bh_base *> _news;
     std::set<bh_base *> _frees;
+    /* Bases released after the kernel without being temporaries of this loop: frees that a
+     * transformer lifted out of the nested loop that held them (a base made in one nested loop and
+     * read in a sibling must stay an array). */
+    std::set<bh_base *> _released;
     bool _reshapable = false;
 
     LoopB() = default;

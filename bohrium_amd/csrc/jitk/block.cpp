@@ -105,6 +105,7 @@ std::set<bh_base *> LoopB::getAllNews() const {
 
 void LoopB::getAllFrees(std::set<bh_base *> &out) const {
     out.insert(_frees.begin(), _frees.end());
+    out.insert(_released.begin(), _released.end());
     for (const Block &b : _block_list)
         if (!b.isInstr()) b.getLoop().getAllFrees(out);
 }
@@ -316,6 +317,7 @@ LoopB merge(const LoopB &l1, const LoopB &l2) {
     ret._sweeps.insert(l2._sweeps.begin(), l2._sweeps.end());
     ret._news.insert(l2._news.begin(), l2._news.end());
     ret._frees.insert(l2._frees.begin(), l2._frees.end());
+    ret._released.insert(l2._released.begin(), l2._released.end());
     ret._reshapable = is_reshapeable(allInstr(ret));
     return ret;
 }
@@ -414,16 +416,18 @@ Block reshape_and_merge(const LoopB &l1, const LoopB &l2) {
     if (l1.size == l2.size) return Block(merge(l1, l2));
     if (l2._reshapable && l2.size % l1.size == 0) return Block(merge(l1, reshape_loop(l2, l1.size).getLoop()));
     if (l1._reshapable && l1.size % l2.size == 0) return Block(merge(reshape_loop(l1, l2.size).getLoop(), l2));
+    // A block of frees only: its frees are released with the other block, never made temporaries
+    // of it (the freed bases may come from loops nested in it).
     if (allInstr(l1).empty()) {
         LoopB ret = l2;
         auto f = l1.getAllFrees();
-        ret._frees.insert(f.begin(), f.end());
+        ret._released.insert(f.begin(), f.end());
         return Block(std::move(ret));
     }
     if (allInstr(l2).empty()) {
         LoopB ret = l1;
         auto f = l2.getAllFrees();
-        ret._frees.insert(f.begin(), f.end());
+        ret._released.insert(f.begin(), f.end());
         return Block(std::move(ret));
     }
     throw std::runtime_error("reshape_and_merge: the blocks are not mergeable");

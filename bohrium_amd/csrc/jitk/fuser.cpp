@@ -508,8 +508,11 @@ std::vector<InstrPtr> swap_axis(const std::vector<InstrPtr> &list, int64_t a1, i
 }
 
 // The parent is split into one loop per sub-block; the swapped child is rebuilt from its
-// instructions, so every free of its subtree is kept on the rebuilt loop, and the parent's
-// own frees go to the last loop of the split (after every use inside the parent).
+// instructions, one nested loop per instruction of higher rank.  The child's own frees stay
+// frees of the rebuilt loop (their bases are made and used at the child's level); the frees of
+// its nested loops are kept only as releases, since the instructions that made and read those
+// bases may now sit in different nested loops.  The parent's own frees go to the last loop of
+// the split (after every use inside the parent).
 std::vector<Block> swap_blocks(const LoopB &parent, const LoopB *child) {
     std::vector<Block> ret;
     for (const Block &b : parent._block_list) {
@@ -522,7 +525,9 @@ std::vector<Block> swap_blocks(const LoopB &parent, const LoopB *child) {
             loop.size = child->size;
             const auto t = swap_axis(allInstr(*child), parent.rank, child->rank);
             loop._block_list.push_back(create_nested_block(t, child->rank, parent.size));
-            loop._frees = child->getAllFrees();
+            loop._frees = child->_frees;
+            for (bh_base *f : child->getAllFrees())
+                if (!child->_frees.count(f)) loop._released.insert(f);
         }
         ret.emplace_back(std::move(loop));
     }
@@ -575,6 +580,7 @@ bool collapse_loop_with_child(LoopB &loop) {
         if (!child.isInstr() && child.getLoop()._sweeps.empty() && child.getLoop()._reshapable) {
             loop.size *= child.getLoop().size;
             loop._frees.insert(child.getLoop()._frees.begin(), child.getLoop()._frees.end());
+            loop._released.insert(child.getLoop()._released.begin(), child.getLoop()._released.end());
             auto tmp = std::move(child.getLoop()._block_list);
             loop._block_list = std::move(tmp);
             return collapse_instr_axes(loop, loop.rank);

@@ -312,7 +312,8 @@ void EngineHip::dumpLaunch(const char *dir, const KernelPlan &plan, const std::v
         std::fprintf(f, "param %s %s ", plan.param_types[i].c_str(), plan.param_names[i].c_str());
         if (a.kind == ArgSpec::BASE_PTR) {
             const bh_base *b = ord2base[a.base_ordinal];
-            std::fprintf(f, "base %lld %d\n", static_cast<long long>(b->nelem()), bh_type_size(b->dtype()));
+            std::fprintf(f, "base %lld %d %d\n", static_cast<long long>(b->nelem()), bh_type_size(b->dtype()),
+                         a.base_ordinal);
         } else if (a.kind == ArgSpec::GRID_U32) {
             const uint32_t g = static_cast<uint32_t>(plan.grid);
             std::fprintf(f, "bytes %u %u %u %u\n", g & 255u, (g >> 8) & 255u, (g >> 16) & 255u, g >> 24);
