@@ -716,8 +716,40 @@ std::vector<Block> get_block_list(const std::vector<bh_instruction *> &instr_lis
         if (t == "greedy") {
             fuser_greedy(config, block_list, analytic);
         } else if (t == "learned") {
-            if (!fuser_greedy_learned(config, block_list))
-                fuser_greedy(config, block_list, config.learned_weight ? config.learned_weight : analytic);
+            // The model scores whole partitions: the analytic greedy one, the one its own greedy
+            // merging builds and the fully merged (serial) one; the lowest predicted total wins
+            // (ties keep the analytic partition).  Without a model: the analytic greedy fuser.
+            std::vector<Block> greedy = block_list;
+            fuser_greedy(config, greedy, config.learned_weight ? config.learned_weight : analytic);
+            std::vector<Block> learned = block_list;
+            if (!fuser_greedy_learned(config, learned)) {
+                block_list = std::move(greedy);
+                continue;
+            }
+            std::vector<Block> serial = block_list;
+            fuser_serial(serial, config);
+            std::vector<std::vector<Block> *> cands{&greedy, &learned, &serial};
+            std::vector<std::vector<InstrPtr> > sets;
+            std::vector<size_t> owner;
+            for (size_t c = 0; c < cands.size(); ++c)
+                for (const Block &b : *cands[c]) {
+                    std::vector<InstrPtr> in = b.isInstr() ? std::vector<InstrPtr>{b.getInstr()} : allInstr(b.getLoop());
+                    std::vector<InstrPtr> work;
+                    for (const InstrPtr &i : in)
+                        if (!bh_opcode_is_system(i->opcode)) work.push_back(i);
+                    if (work.empty()) continue;
+                    sets.push_back(std::move(work));
+                    owner.push_back(c);
+                }
+            std::vector<double> cost;
+            size_t pick = 0;
+            if (config.learned_cost(sets, cost) && cost.size() == sets.size()) {
+                std::vector<double> total(cands.size(), 0.0);
+                for (size_t k = 0; k < sets.size(); ++k) total[owner[k]] += cost[k];
+                for (size_t c = 1; c < cands.size(); ++c)
+                    if (total[c] < total[pick]) pick = c;
+            }
+            block_list = std::move(*cands[pick]);
         } else if (t == "serial") {
             fuser_serial(block_list, config);
         } else if (t == "breadth_first") {

@@ -103,3 +103,27 @@ def test_hip_adam_training_tracks_oracle():
     np.testing.assert_allclose(losses, ref, rtol=2e-3, atol=1e-5)
     more = m.train(x, t, 400, lr=1e-2)
     assert more[-1] < 0.5 * losses[0]
+
+
+@pytest.mark.gpu
+def test_learned_fuser_with_the_shipped_model():
+    """With the model trained on this device (bohrium_amd/data/costmodel_mi355x.npz, made by
+    bohrium_amd.costmodel_train from programs 100-129), the learned fuser's partitions of held-out
+    programs run within 10 % of the better of the analytic greedy and serial fusers, and well below
+    the serial fuser, whose all-in-one blocks it scores as slower."""
+    from bohrium_amd import _bh_api as api
+    from bohrium_amd import costmodel
+    from bohrium_amd import costmodel_train as ct
+    model = costmodel.CostModel.load(ct.DEFAULT_MODEL)
+    model.install()
+    api.set_flush_threshold(10 ** 7)
+    try:
+        rows, tot, best = ct.evaluate([1003, 1005, 1006], pipelines=("greedy", "serial", "learned"),
+                                      sides=(512, 1024), reps=2)
+        print("device ms", tot, "per-program best", best)
+        assert tot["learned"] <= 1.10 * min(tot["greedy"], tot["serial"])
+        assert tot["learned"] < 0.8 * tot["serial"]
+    finally:
+        costmodel.uninstall()
+        api.set_flush_threshold(1000)
+        api.message("fuser_list: " + ct.PIPELINES["greedy"])
