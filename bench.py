@@ -13,7 +13,7 @@ Algorithmic bytes per time step = read S, K, T once = 24 * N bytes.
 Prints one JSON line with the value (GB/s over the whole timed job), the roofline
 of the dominant kernel (per-launch device time from HIP events on the engine
 stream), the CPU baseline (the oracle's fused OpenMP form on the host cores) and
-secondary measurements of the 2-D Jacobi and batched-arrays workloads.
+secondary measurements of the 2-D Jacobi and batched-arrays workloads and of the MFMA GEMM.
 """
 import argparse
 import json
@@ -137,6 +137,31 @@ def batched_secondary(bh, workloads, api):
     return out
 
 
+def gemm_secondary(bh):
+    """blas_gemm (the MFMA extension method) on 8192 x 8192 operands, fp64 and fp32: wall time per
+    product between host syncs, operands resident."""
+    import numpy as np
+    out = {}
+    n = 8192
+    for name, dt, peak in (("f64", np.float64, 78.6), ("f32", np.float32, 157.3)):
+        R = bh.random.RandomState(7)
+        a = R.random_sample((n, n)).astype(dt)
+        b = R.random_sample((n, n)).astype(dt)
+        bh.asnumpy_many([a[:1, :1], b[:1, :1]])
+        bh.blas.gemm(a, b)[:1, :1].asnumpy()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            c = bh.blas.gemm(a, b)
+            c[:1, :1].asnumpy()
+            ts.append(time.perf_counter() - t0)
+            del c
+        tf = 2.0 * n ** 3 / min(ts) / 1e12
+        out[name] = {"n": n, "ms": round(min(ts) * 1e3, 3), "TFLOPs": round(tf, 1), "frac_of_mfma_peak": round(tf / peak, 3)}
+        del a, b
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -193,7 +218,8 @@ def main():
     secondary = None
     if not args.no_secondary:
         del S, K, T
-        secondary = {"jacobi": jacobi_secondary(bh, workloads), "batched": batched_secondary(bh, workloads, api)}
+        secondary = {"jacobi": jacobi_secondary(bh, workloads), "batched": batched_secondary(bh, workloads, api),
+                     "blas_gemm": gemm_secondary(bh)}
 
     cpu = None
     if not args.no_cpu:

@@ -90,8 +90,13 @@ def main(argv=None):
     ap.add_argument("--train", default="100:130")
     ap.add_argument("--test", default="1000:1012")
     ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--eval-only", action="store_true", help="evaluate the model at --out without training")
     a = ap.parse_args(argv)
     api.set_flush_threshold(10 ** 7)
+    if a.eval_only:
+        costmodel.CostModel.load(a.out).install()
+        _report(a)
+        return
     lo, hi = (int(v) for v in a.train.split(":"))
     t0 = time.time()
     X, Y = collect(range(lo, hi))
@@ -107,6 +112,10 @@ def main(argv=None):
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     model.save(a.out)
     model.install()
+    _report(a)
+
+
+def _report(a):
     lo, hi = (int(v) for v in a.test.split(":"))
     _, tot, best = evaluate(range(lo, hi))
     print("total device ms", {k: round(v, 3) for k, v in tot.items()}, "per-program best %.3f" % best)
