@@ -126,10 +126,8 @@ struct Tile {
 // C = op(A) op(B); op(A)(m, k) = TA ? A[k * lda + m] : A[m * lda + k],
 // op(B)(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n].  UPPER: only C(m, n) with m <= n is stored.
 // vec_ok: lda and ldb are multiples of the 16-byte vector and A, B are 16-byte aligned.
-// (at least two waves per SIMD: the fp64 4 x 4 tile otherwise takes 136 VGPRs + 128 AGPRs and
-// runs one wave per SIMD, which idles the matrix core through every barrier and LDS phase)
 template <typename T, bool TA, bool TB, bool UPPER, int TM, int TN>
-__global__ void __launch_bounds__(kThreads, 2)
+__global__ void __launch_bounds__(kThreads)
 gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, int64_t M, int64_t N, int64_t K,
           int64_t lda, int64_t ldb, int64_t ldc, int vec_ok) {
     constexpr int BM = 32 * TM, BN = 32 * TN;
