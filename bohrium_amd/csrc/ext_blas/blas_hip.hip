@@ -2,12 +2,11 @@
  * the reference's blas_* extension methods (include/bh_blas_hip.h).
  *
  * Real types: one workgroup of 4 waves computes a BM x BN tile of C; wave w owns
- * the (w / 2, w % 2) quarter as TM x TN MFMA tiles of 16 x 16.  Per K step of
- * 16 the workgroup stages op(A) (BM x 16) and op(B) (16 x BN) in LDS, k-major,
- * rows padded by 16 elements so that the 16-lane groups of one MFMA operand
- * read (lanes 0-15: row k, lanes 16-31: row k + 1, ...) fall in disjoint banks.
- * The next K step is fetched into registers while the current one runs on the
- * matrix cores.  MFMA operand layout (16 x 16 x 4, measured on gfx950): lane l
+ * the (w / 2, w % 2) quarter as TM x TN MFMA tiles of 16 x 16.  Per K step (BK
+ * of 16 or 32) the workgroup stages op(A) (BM x BK) and op(B) (BK x BN) in LDS
+ * with padded rows so that the 16-lane groups of one MFMA operand read fall in
+ * disjoint banks; the LDS tiles are double-buffered and the next K step is
+ * fetched into registers while the current one runs on the matrix cores.  MFMA operand layout (16 x 16 x 4, measured on gfx950): lane l
  * supplies A[l % 16][l / 16] and B[l / 16][l % 16]; its 4 results are
  * D[4 (l / 16) + r][l % 16] for f32 and D[4 r + l / 16][l % 16] for f64, r = 0..3.
  *
@@ -16,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -48,7 +48,6 @@ template <> struct Mfma<float> {
 };
 
 constexpr int kThreads = 256;
-constexpr int kBK = 16;
 constexpr int kPad = 16;
 
 template <typename T> struct Vec;
@@ -61,13 +60,13 @@ template <> struct Vec<float> {
     static constexpr int n = 4;
 };
 
-// One operand tile: ROWS rows (m of op(A) or n of op(B)) by kBK steps of k.  KCONT: k is the
+// One operand tile: ROWS rows (m of op(A) or n of op(B)) by kBK steps of k (16 or 32).  KCONT: k is the
 // contiguous dimension in memory (element (r, k) at P[r * ld + k]); otherwise r is
 // (element at P[k * ld + r]).  A thread moves chunks of V elements along the contiguous dimension
 // (one 16-byte load on interior tiles).  LDS layout: [ROWS][kBK + KPAD] when KCONT, else
 // [kBK][ROWS + kPad]; with these pads the 16-lane groups of an MFMA operand read (16 rows at one k,
 // the next group at k + 1) and the chunk stores fall in disjoint banks.
-template <typename T, bool KCONT, int ROWS>
+template <typename T, bool KCONT, int ROWS, int kBK>
 struct Tile {
     static constexpr int V = Vec<T>::n;
     static constexpr int PER = ROWS * kBK / kThreads;
@@ -126,15 +125,17 @@ struct Tile {
 // C = op(A) op(B); op(A)(m, k) = TA ? A[k * lda + m] : A[m * lda + k],
 // op(B)(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n].  UPPER: only C(m, n) with m <= n is stored.
 // vec_ok: lda and ldb are multiples of the 16-byte vector and A, B are 16-byte aligned.
-template <typename T, bool TA, bool TB, bool UPPER, int TM, int TN>
+// The LDS tiles are double-buffered: while the matrix cores work on one K step, the next one is
+// loaded into registers and then stored into the other buffer; one barrier per K step.
+template <typename T, bool TA, bool TB, bool UPPER, int TM, int TN, int BK>
 __global__ void __launch_bounds__(kThreads)
 gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, int64_t M, int64_t N, int64_t K,
           int64_t lda, int64_t ldb, int64_t ldc, int vec_ok) {
     constexpr int BM = 32 * TM, BN = 32 * TN;
-    typedef Tile<T, !TA, BM> ATile;
-    typedef Tile<T, TB, BN> BTile;
-    __shared__ __attribute__((aligned(16))) T As[ATile::LDS];
-    __shared__ __attribute__((aligned(16))) T Bs[BTile::LDS];
+    typedef Tile<T, !TA, BM, BK> ATile;
+    typedef Tile<T, TB, BN, BK> BTile;
+    __shared__ __attribute__((aligned(16))) T As[2][ATile::LDS];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BTile::LDS];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t bm = static_cast<int64_t>(blockIdx.y) * BM, bn = static_cast<int64_t>(blockIdx.x) * BN;
@@ -152,31 +153,40 @@ gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, i
 
     T ra[ATile::PER], rb[BTile::PER];
     auto fetch = [&](int64_t k0) {
-        const bool kin = k0 + kBK <= K;
+        const bool kin = k0 + BK <= K;
         ATile::fetch(A, lda, bm, k0, M, K, a_in && kin, ra);
         BTile::fetch(B, ldb, bn, k0, N, K, b_in && kin, rb);
     };
 
     fetch(0);
-    for (int64_t k0 = 0; k0 < K; k0 += kBK) {
-        __syncthreads();  // every wave is done reading the previous tile
-        ATile::store(As, ra);
-        BTile::store(Bs, rb);
-        __syncthreads();
-        if (k0 + kBK < K) fetch(k0 + kBK);
+    ATile::store(As[0], ra);
+    BTile::store(Bs[0], rb);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) fetch(k0 + BK);
+        const T *as = As[buf], *bs = Bs[buf];
 #pragma unroll
-        for (int ks = 0; ks < kBK; ks += 4) {
+        for (int ks = 0; ks < BK; ks += 4) {
             const int kr = ks + (lane >> 4);
             T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = ATile::read(As, wm + i * 16 + (lane & 15), kr);
+            for (int i = 0; i < TM; ++i) a[i] = ATile::read(as, wm + i * 16 + (lane & 15), kr);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = BTile::read(Bs, wn + j * 16 + (lane & 15), kr);
+            for (int j = 0; j < TN; ++j) b[j] = BTile::read(bs, wn + j * 16 + (lane & 15), kr);
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::step(a[i], b[j], acc[i][j]);
         }
+        if (more) {
+            ATile::store(As[buf ^ 1], ra);
+            BTile::store(Bs[buf ^ 1], rb);
+        }
+        // the other buffer is complete, and every wave is done reading this one before it is refilled
+        __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -231,16 +241,28 @@ void launch_real(int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, co
     const int64_t v = Vec<T>::n;
     const int vec_ok = lda % v == 0 && ldb % v == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
                        reinterpret_cast<uintptr_t>(B) % 16 == 0;
-    // 128 x 128 tiles when the matrix fills the device with them, 64 x 64 otherwise.
+    // 128 x 128 tiles when the matrix fills the device with them, 64 x 64 otherwise; K steps of
+    // BH_BLAS_BK elements (16 or 32; 16 measured faster on MI355X: 8192^3 fp64 59.5 vs 40.6,
+    // fp32 129.1 vs 120.0 TFLOP/s).
+    static const int bk = [] {
+        const char *e = std::getenv("BH_BLAS_BK");
+        return e != nullptr && std::atoi(e) == 32 ? 32 : 16;
+    }();
     const bool big = ((m + 127) / 128) * ((n + 127) / 128) >= 512;
+    const T *a = static_cast<const T *>(A), *b = static_cast<const T *>(B);
+    T *c = static_cast<T *>(C);
     if (big) {
         const dim3 grid(static_cast<unsigned>((n + 127) / 128), static_cast<unsigned>((m + 127) / 128));
-        gemm_mfma<T, TA, TB, UPPER, 4, 4><<<grid, kThreads, 0, s>>>(static_cast<const T *>(A), static_cast<const T *>(B),
-                                                                    static_cast<T *>(C), m, n, k, lda, ldb, ldc, vec_ok);
+        if (bk == 32)
+            gemm_mfma<T, TA, TB, UPPER, 4, 4, 32><<<grid, kThreads, 0, s>>>(a, b, c, m, n, k, lda, ldb, ldc, vec_ok);
+        else
+            gemm_mfma<T, TA, TB, UPPER, 4, 4, 16><<<grid, kThreads, 0, s>>>(a, b, c, m, n, k, lda, ldb, ldc, vec_ok);
     } else {
         const dim3 grid(static_cast<unsigned>((n + 63) / 64), static_cast<unsigned>((m + 63) / 64));
-        gemm_mfma<T, TA, TB, UPPER, 2, 2><<<grid, kThreads, 0, s>>>(static_cast<const T *>(A), static_cast<const T *>(B),
-                                                                    static_cast<T *>(C), m, n, k, lda, ldb, ldc, vec_ok);
+        if (bk == 32)
+            gemm_mfma<T, TA, TB, UPPER, 2, 2, 32><<<grid, kThreads, 0, s>>>(a, b, c, m, n, k, lda, ldb, ldc, vec_ok);
+        else
+            gemm_mfma<T, TA, TB, UPPER, 2, 2, 16><<<grid, kThreads, 0, s>>>(a, b, c, m, n, k, lda, ldb, ldc, vec_ok);
     }
 }
 
