@@ -205,3 +205,23 @@ def jacobi_repeat(G, iters):
     from . import do_while
     do_while(body, iters)
     return delta
+
+
+if __name__ == "__main__":
+    # Small drivers for profiling runs:
+    #   python -m bohrium_amd.workloads jacobi [n] [sweeps]       (default 8192, 5)
+    #   python -m bohrium_amd.workloads black_scholes [n] [steps] (default 20000000, 10)
+    import sys
+    what = sys.argv[1] if len(sys.argv) > 1 else "jacobi"
+    if what == "jacobi":
+        n = int(sys.argv[2]) if len(sys.argv) > 2 else 8192
+        it = int(sys.argv[3]) if len(sys.argv) > 3 else 5
+        grid = jacobi_grid(n)
+        print("jacobi %d x %d, %d sweeps: last max |delta| %r" % (n, n, it, float(jacobi_repeat(grid, it).asnumpy())))
+    elif what == "black_scholes":
+        n = int(sys.argv[2]) if len(sys.argv) > 2 else 20_000_000
+        st = int(sys.argv[3]) if len(sys.argv) > 3 else 10
+        S, K, T = black_scholes_inputs(n)
+        print("black-scholes n=%d: sums %s" % (n, black_scholes_repeat(S, K, T, st).asnumpy()[:3]))
+    else:
+        raise SystemExit("usage: python -m bohrium_amd.workloads jacobi|black_scholes [n] [steps]")
