@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _bh_api
 from . import opcodes
-from .bharray import (BhArray, array, empty, full, zeros, ones, arange, take, put, flush, ufunc,
+from .bharray import (BhArray, array, empty, full, zeros, ones, empty_like, zeros_like, ones_like, arange, take, put, flush, ufunc,
                       assign, asnumpy_many, Iterator, get_iterator)
 from . import random
 from . import user_kernel

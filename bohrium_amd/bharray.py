@@ -91,8 +91,15 @@ class BhArray:
         return "BhArray(%r, dtype=%s)" % (self.asnumpy(), self.dtype)
 
     # ------------------------------------------------------------ data
+    def _small_view(self):
+        """A view of at most 1/8 of a large base: fetching it copies it to a base of its own on the
+        device first, so only its elements cross to the host, not the whole base."""
+        return 0 < self.size and self.base.nelem >= 65536 and self.size * 8 <= self.base.nelem
+
     def asnumpy(self):
         """Flush and return a NumPy copy of this view."""
+        if self._small_view():
+            return self.copy().asnumpy()
         api.sync(self.base.handle)
         api.flush()
         host = api.data_copy_out(self.base.handle, self.dtype, max(1, self.base.nelem))
@@ -411,6 +418,18 @@ def ones(shape, dtype=np.float64):
     return full(shape, 1, dtype)
 
 
+def empty_like(a, dtype=None):
+    return empty(a.shape, a.dtype if dtype is None else dtype)
+
+
+def zeros_like(a, dtype=None):
+    return zeros(a.shape, a.dtype if dtype is None else dtype)
+
+
+def ones_like(a, dtype=None):
+    return ones(a.shape, a.dtype if dtype is None else dtype)
+
+
 def arange(start, stop=None, step=1, dtype=None):
     """Evenly spaced values (BH_RANGE, then an affine map when start/step are not 0/1)."""
     if stop is None:
@@ -436,7 +455,8 @@ def flush():
 
 
 def asnumpy_many(arrays):
-    """Fetch several arrays with a single flush."""
+    """Fetch several arrays with a single flush (small views of large bases through copies of their own)."""
+    arrays = [a.copy() if a._small_view() else a for a in arrays]
     for a in arrays:
         api.sync(a.base.handle)
     api.flush()

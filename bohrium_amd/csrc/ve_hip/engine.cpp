@@ -642,8 +642,11 @@ void EngineHip::handleExtmethod(BhIR *bhir) {
             };
             ctx.stream = dry_run ? nullptr : bhhip_stream();
             ext->second.execute(&instr, &ctx);
-            if (!instr.operand.empty() && !instr.operand[0].isConstant()) {
-                DeviceBuffer &d = buffer(instr.operand[0].base);
+            // An extension method may write any of its operands (blas_trmm/blas_trsm write B, not C):
+            // the device copies of all of them are now the current ones.
+            for (const bh_view &v : instr.operand) {
+                if (v.isConstant()) continue;
+                DeviceBuffer &d = buffer(v.base);
                 d.dev_valid = true;
                 d.host_valid = false;
             }

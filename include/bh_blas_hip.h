@@ -7,19 +7,31 @@
  * the reference's CBLAS-backed extmethods/blas (templates/body.tpl:1-50,
  * body_func.tpl:1-30, methods.json) for the methods below, on the device:
  *
- *   blas_gemm    C := A B      A m x k, B k x n, C m x n   (body_func.tpl: alpha 1, beta 0)
- *   blas_gemmt   C := A^T B    A m x k, B m x n, C k x n
- *   blas_syrk    C := A A^T    upper triangle of C (m x m), the strictly lower part is left as it was
+ *   blas_gemm    C := A B          A m x k, B k x n, C m x n   (body_func.tpl: alpha 1, beta 0)
+ *   blas_gemmt   C := A^T B        A m x k, B m x n, C k x n
+ *   blas_symm    C := A B          A m x m symmetric, read from its upper triangle
+ *   blas_hemm    C := A B          A m x m Hermitian, upper triangle, diagonal taken as real (complex)
+ *   blas_syrk    C := A A^T        upper triangle of C (n x n), A n x k
+ *   blas_herk    C := A A^H        upper triangle, real diagonal (complex)
+ *   blas_syr2k   C := A B^T + B A^T  upper triangle, A and B n x k
+ *   blas_her2k   C := A B^H + B A^H  upper triangle, real diagonal (complex)
+ *   blas_trmm    B := A B          A m x m unit upper triangular (diagonal and lower part not read)
+ *   blas_trsm    B := A^-1 B       A m x m unit upper triangular
  *
- * Instruction operands are (C, A, B) for gemm/gemmt and (C, A) for syrk, as the
- * reference front end queues them (bridge/npbackend/bohrium/blas.py:12-45); the
+ * The k-update methods write only C's upper triangle; its strictly lower part
+ * keeps what C held.  Instruction operands are (C, A, B) and (C, A) for
+ * syrk/herk, as the reference front end queues them
+ * (bridge/npbackend/bohrium/blas.py:12-117); trmm/trsm write B and ignore C; the
  * front end applies alpha and beta before the call.  Matrices are row-major and
  * C-contiguous; float32 and float64 run on MFMA (v_mfma_f32_16x16x4f32,
  * v_mfma_f64_16x16x4f64), complex64/complex128 on a tiled FMA kernel.
  *
- * bhblas_gemm is the same device routine as a plain C entry point (device
- * pointers, the engine stream or any HIP stream) for callers outside a
- * component stack and for benchmarks.
+ * The bhblas_* functions are the same device routines as plain C entry points
+ * (device pointers, leading dimensions in elements, the engine stream or any
+ * HIP stream; they return without waiting, 0 on success, else
+ * bhblas_last_error() says why) for callers outside a component stack and for
+ * benchmarks.  symm/hemm/trmm use a stream-ordered device temporary of m x m
+ * (trmm: also m x n) elements.
  */
 #ifndef BH_BLAS_HIP_H
 #define BH_BLAS_HIP_H
@@ -45,16 +57,49 @@ enum { BHBLAS_F32 = 0, BHBLAS_F64 = 1, BHBLAS_C64 = 2, BHBLAS_C128 = 3 };
  * (a hipStream_t; NULL = the default stream) and returns without waiting.  0 on success. */
 int bhblas_gemm(int dtype, int trans_a, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda,
                 const void *B, int64_t ldb, void *C, int64_t ldc, void *stream);
-/* The upper triangle (i <= j) of C = A * A^T, A m x k, C m x m. */
+/* The upper triangle (i <= j) of C = A * A^T, A m x k, C m x m (cblas_?syrk, methods.json "syrk"). */
 int bhblas_syrk(int dtype, int64_t m, int64_t k, const void *A, int64_t lda, void *C, int64_t ldc, void *stream);
+/* The upper triangle of C = A * A^H with a zero imaginary diagonal; complex only (cblas_?herk). */
+int bhblas_herk(int dtype, int64_t n, int64_t k, const void *A, int64_t lda, void *C, int64_t ldc, void *stream);
+/* The upper triangle of C = A B^T + B A^T, A and B n x k (cblas_?syr2k). */
+int bhblas_syr2k(int dtype, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                 int64_t ldc, void *stream);
+/* The upper triangle of C = A B^H + B A^H with a zero imaginary diagonal; complex only (cblas_?her2k). */
+int bhblas_her2k(int dtype, int64_t n, int64_t k, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                 int64_t ldc, void *stream);
+/* C = A B, A m x m symmetric given by its upper triangle, B and C m x n (cblas_?symm, left, upper). */
+int bhblas_symm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                int64_t ldc, void *stream);
+/* C = A B, A m x m Hermitian given by its upper triangle; complex only (cblas_?hemm, left, upper). */
+int bhblas_hemm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                int64_t ldc, void *stream);
+/* B := A B in place, A m x m unit upper triangular, B m x n (cblas_?trmm, left, upper, no-trans, unit). */
+int bhblas_trmm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
+/* B := A^-1 B in place (solves A X = B), A m x m unit upper triangular (cblas_?trsm, same options). */
+int bhblas_trsm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
 const char *bhblas_last_error(void);
 
+/* Extension methods blas_<name> (templates/methods.json "methods"). */
 bh_extmethod_impl *blas_gemm_create(void);
 void blas_gemm_destroy(bh_extmethod_impl *self);
 bh_extmethod_impl *blas_gemmt_create(void);
 void blas_gemmt_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_symm_create(void);
+void blas_symm_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_hemm_create(void);
+void blas_hemm_destroy(bh_extmethod_impl *self);
 bh_extmethod_impl *blas_syrk_create(void);
 void blas_syrk_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_herk_create(void);
+void blas_herk_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_syr2k_create(void);
+void blas_syr2k_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_her2k_create(void);
+void blas_her2k_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_trmm_create(void);
+void blas_trmm_destroy(bh_extmethod_impl *self);
+bh_extmethod_impl *blas_trsm_create(void);
+void blas_trsm_destroy(bh_extmethod_impl *self);
 
 #ifdef __cplusplus
 }
