@@ -30,6 +30,8 @@ namespace {
 const char *kPreludeRef() {
     static const char *p =
 #include "kernel_prelude.inc"
+        "\n#define BH_MATH_FN __device__ __forceinline__\n"
+#include "kernel_math.inc"
         ;
     return p;
 }

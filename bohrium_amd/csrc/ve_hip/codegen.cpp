@@ -78,7 +78,7 @@ std::string cast_expr(bh_type to, bh_type from, const std::string &e) {
 // Scalar body of one instruction; `ops` are the rendered operands in the
 // reference's order (out, in1, in2, ...).  Semantics follow the reference's
 // write_operation() for C99 (core/jitk/instruction.cpp:58-544).
-std::string hip_operation(const bh_instruction &instr, const std::vector<std::string> &ops) {
+std::string hip_operation(const bh_instruction &instr, const std::vector<std::string> &ops, bool math64) {
     std::ostringstream o;
     const auto &a = ops;
     auto bin = [&](const char *op) { o << a[0] << " = " << a[1] << " " << op << " " << a[2] << ";"; };
@@ -182,7 +182,7 @@ std::string hip_operation(const bh_instruction &instr, const std::vector<std::st
         case BH_SINH: un("sinh", "bh_csinh"); break;
         case BH_COSH: un("cosh", "bh_ccosh"); break;
         case BH_TANH: un("tanh", "bh_ctanh"); break;
-        case BH_EXP: un("exp", "bh_cexp"); break;
+        case BH_EXP: un(math64 && instr.operand_type(1) == bh_type::FLOAT64 ? "bh_exp64" : "exp", "bh_cexp"); break;
         case BH_ABSOLUTE: {
             const bh_type t = instr.operand_type(1);
             if (t == bh_type::BOOL || bh_type_is_unsigned_integer(t)) {
@@ -199,7 +199,7 @@ std::string hip_operation(const bh_instruction &instr, const std::vector<std::st
             break;
         }
         case BH_SQRT: un("sqrt", "bh_csqrt"); break;
-        case BH_LOG: un("log", "bh_clog"); break;
+        case BH_LOG: un(math64 && instr.operand_type(1) == bh_type::FLOAT64 ? "bh_log64" : "log", "bh_clog"); break;
         case BH_NOT_EQUAL: bin("!="); break;
         case BH_EQUAL: bin("=="); break;
         case BH_POWER:
@@ -676,7 +676,7 @@ void Writer::emit_instr(const bh_instruction &instr, Scope &sc, int ind) {
             }
         }
     }
-    body << hip_operation(instr, ops) << "\n";
+    body << hip_operation(instr, ops, cfg.math64) << "\n";
 }
 
 std::vector<KernelParam> Writer::parameters() {

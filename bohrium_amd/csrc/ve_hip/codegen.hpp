@@ -100,6 +100,8 @@ struct WriterConfig {
     /* Grid-stride skeletons (MAP, FULL_REDUCE, ROW_REDUCE) launch at most the workgroups that are
      * resident on the device at once (CUs x occupancy of the compiled kernel). */
     bool resident_grid = false;
+    /* float64 exp and log through the prelude's bh_exp64 / bh_log64 (false: the HIP math library's). */
+    bool math64 = true;
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
 

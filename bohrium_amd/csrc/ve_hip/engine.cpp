@@ -19,6 +19,8 @@ namespace vehip {
 namespace {
 const char *kPrelude =
 #include "kernel_prelude.inc"
+    "\n#define BH_MATH_FN __device__ __forceinline__\n"
+#include "kernel_math.inc"
     ;
 
 void check(int rc, const char *what) {
@@ -984,7 +986,8 @@ public:
                 if (eq == std::string::npos) throw std::invalid_argument("hip engine: bad codegen setting: " + kv);
                 const std::string k = kv.substr(0, eq);
                 const int v = std::stoi(kv.substr(eq + 1));
-                if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu" && k != "resident_grid"))
+                if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu" && k != "resident_grid" &&
+                              k != "math64"))
                     throw std::invalid_argument("hip engine: codegen setting must be >= 1: " + kv);
                 if (k == "max_grid") {
                     engine.codegenConfig().max_grid = v;
@@ -994,6 +997,8 @@ public:
                     engine.codegenConfig().reduce_unroll = v;
                 } else if (k == "resident_grid") {
                     engine.codegenConfig().resident_grid = v != 0;
+                } else if (k == "math64") {
+                    engine.codegenConfig().math64 = v != 0;
                 } else if (k == "reduce_max_grid") {
                     engine.codegenConfig().reduce_max_grid = v;
                 } else if (k == "reduce_min_items") {
@@ -1012,7 +1017,8 @@ public:
             ss << "max_grid=" << w.max_grid << " map_unroll=" << w.map_unroll << " reduce_unroll=" << w.reduce_unroll
                << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items
                << " scan_items=" << w.scan_items << " row_reduce=" << (w.row_reduce ? 1 : 0)
-               << " waves_per_eu=" << w.waves_per_eu << " resident_grid=" << (w.resident_grid ? 1 : 0);
+               << " waves_per_eu=" << w.waves_per_eu << " resident_grid=" << (w.resident_grid ? 1 : 0)
+               << " math64=" << (w.math64 ? 1 : 0);
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
