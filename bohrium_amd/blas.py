@@ -167,6 +167,14 @@ def _bench(n=8192, reps=3):
         tf = 2.0 * n ** 3 / best / 1e12
         print("gemm %s n=%d: %.3f ms, %.1f TFLOP/s (%.0f%% of the %.1f TFLOP/s MFMA peak)"
               % (np.dtype(dt).name, n, best * 1e3, tf, 100 * tf / peak, peak), flush=True)
+        # the triangular solve: n^3 flops (n x n unit upper triangular A, n right-hand sides)
+        def solve():
+            return (trsm(a, b.copy())[:1, :1] * 1.0).asnumpy()
+        solve()
+        t0 = time.perf_counter()
+        solve()
+        ts = time.perf_counter() - t0
+        print("trsm %s n=%d: %.3f ms, %.1f TFLOP/s" % (np.dtype(dt).name, n, ts * 1e3, n ** 3 / ts / 1e12), flush=True)
 
 
 if __name__ == "__main__":

@@ -123,13 +123,14 @@ struct Tile {
     }
 };
 
-// C = alpha op(A) op(B) + beta C (C is not read when beta is 0); op(A)(m, k) = TA ? A[k * lda + m] :
-// A[m * lda + k], op(B)(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n].  UPPER: only C(m, n) with m <= n
-// is stored.
+// C = op(A) op(B), or alpha op(A) op(B) + beta C when SCALED (C is not read when beta is 0); op(A)(m, k)
+// = TA ? A[k * lda + m] : A[m * lda + k], op(B)(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n].  UPPER:
+// only C(m, n) with m <= n is stored.  The scaled epilogue is a separate instantiation: in the plain
+// product it costs the f64 128 x 128 kernel ~120 VGPRs and half its throughput.
 // vec_ok: lda and ldb are multiples of the 16-byte vector and A, B are 16-byte aligned.
 // The LDS tiles are double-buffered: while the matrix cores work on one K step, the next one is
 // loaded into registers and then stored into the other buffer; one barrier per K step.
-template <typename T, bool TA, bool TB, bool UPPER, int TM, int TN, int BK>
+template <typename T, bool TA, bool TB, bool UPPER, int TM, int TN, int BK, bool SCALED>
 __global__ void __launch_bounds__(kThreads)
 gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, int64_t M, int64_t N, int64_t K,
           int64_t lda, int64_t ldb, int64_t ldc, int vec_ok, T alpha, T beta) {
@@ -199,9 +200,13 @@ gemm_mfma(const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, i
                 const int64_t row = bm + wm + i * 16 + Mfma<T>::row(lane, r);
                 const int64_t col = bn + wn + j * 16 + (lane & 15);
                 if (row < M && col < N && (!UPPER || row <= col)) {
-                    T v = alpha * acc[i][j][r];
-                    if (beta != T(0)) v += beta * C[row * ldc + col];
-                    C[row * ldc + col] = v;
+                    if (SCALED) {
+                        T v = alpha * acc[i][j][r];
+                        if (beta != T(0)) v += beta * C[row * ldc + col];
+                        C[row * ldc + col] = v;
+                    } else {
+                        C[row * ldc + col] = acc[i][j][r];
+                    }
                 }
             }
 }
@@ -339,11 +344,8 @@ struct Gemm {
     hipStream_t s;
 };
 
-template <typename T, bool TA, bool TB, bool UPPER>
-void launch_real(const Gemm &g) {
-    const int64_t v = Vec<T>::n;
-    const int vec_ok = g.lda % v == 0 && g.ldb % v == 0 && reinterpret_cast<uintptr_t>(g.A) % 16 == 0 &&
-                       reinterpret_cast<uintptr_t>(g.B) % 16 == 0;
+template <typename T, bool TA, bool TB, bool UPPER, bool SCALED>
+void launch_real_tiles(const Gemm &g, int vec_ok) {
     // 128 x 128 tiles when the matrix fills the device with them, 64 x 64 otherwise; K steps of
     // BH_BLAS_BK elements (16 or 32; 16 measured faster on MI355X: 8192^3 fp64 59.5 vs 40.6,
     // fp32 129.1 vs 120.0 TFLOP/s).
@@ -359,20 +361,36 @@ void launch_real(const Gemm &g) {
     if (big) {
         const dim3 grid(static_cast<unsigned>((n + 127) / 128), static_cast<unsigned>((m + 127) / 128));
         if (bk == 32)
-            gemm_mfma<T, TA, TB, UPPER, 4, 4, 32>
+            gemm_mfma<T, TA, TB, UPPER, 4, 4, 32, SCALED>
                 <<<grid, kThreads, 0, g.s>>>(a, b, c, m, n, g.k, g.lda, g.ldb, g.ldc, vec_ok, al, be);
         else
-            gemm_mfma<T, TA, TB, UPPER, 4, 4, 16>
+            gemm_mfma<T, TA, TB, UPPER, 4, 4, 16, SCALED>
                 <<<grid, kThreads, 0, g.s>>>(a, b, c, m, n, g.k, g.lda, g.ldb, g.ldc, vec_ok, al, be);
     } else {
         const dim3 grid(static_cast<unsigned>((n + 63) / 64), static_cast<unsigned>((m + 63) / 64));
         if (bk == 32)
-            gemm_mfma<T, TA, TB, UPPER, 2, 2, 32>
+            gemm_mfma<T, TA, TB, UPPER, 2, 2, 32, SCALED>
                 <<<grid, kThreads, 0, g.s>>>(a, b, c, m, n, g.k, g.lda, g.ldb, g.ldc, vec_ok, al, be);
         else
-            gemm_mfma<T, TA, TB, UPPER, 2, 2, 16>
+            gemm_mfma<T, TA, TB, UPPER, 2, 2, 16, SCALED>
                 <<<grid, kThreads, 0, g.s>>>(a, b, c, m, n, g.k, g.lda, g.ldb, g.ldc, vec_ok, al, be);
     }
+}
+
+template <typename T, bool TA, bool TB, bool UPPER>
+bool launch_real(const Gemm &g) {
+    const int64_t v = Vec<T>::n;
+    const int vec_ok = g.lda % v == 0 && g.ldb % v == 0 && reinterpret_cast<uintptr_t>(g.A) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(g.B) % 16 == 0;
+    if (g.alpha == 1.0 && g.beta == 0.0) {
+        launch_real_tiles<T, TA, TB, UPPER, false>(g, vec_ok);
+    } else if constexpr (!TA) {  // trsm's updates and syr2k's second pass
+        launch_real_tiles<T, TA, TB, UPPER, true>(g, vec_ok);
+    } else {
+        g_err = "bhblas: scaled product with a transposed A is not provided";
+        return false;
+    }
+    return true;
 }
 
 template <typename R, bool TA, bool TB, bool CONJB, bool UPPER, bool REALDIAG>
@@ -427,8 +445,12 @@ int dispatch(const Gemm &g) {
     if (!check_call(g.dtype, g.m, g.n, g.k)) return 1;
     if (g.m == 0 || g.n == 0) return 0;
     switch (g.dtype) {
-        case BHBLAS_F32: launch_real<float, TA, TB, UPPER>(g); break;
-        case BHBLAS_F64: launch_real<double, TA, TB, UPPER>(g); break;
+        case BHBLAS_F32:
+            if (!launch_real<float, TA, TB, UPPER>(g)) return 1;
+            break;
+        case BHBLAS_F64:
+            if (!launch_real<double, TA, TB, UPPER>(g)) return 1;
+            break;
         case BHBLAS_C64: launch_complex<float, TA, TB, CONJB, UPPER, REALDIAG>(g); break;
         case BHBLAS_C128: launch_complex<double, TA, TB, CONJB, UPPER, REALDIAG>(g); break;
     }
