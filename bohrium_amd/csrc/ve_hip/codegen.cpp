@@ -365,6 +365,7 @@ private:
     std::map<const LoopB *, int> lid;
     std::vector<const LoopB *> loops;
     std::vector<InstrPtr> const_instrs;
+    std::vector<InstrPtr> const_params;  // one instruction per distinct constant: parameter c<j>
     std::map<int64_t, int> const_of_origin;
     std::set<const bh_base *> index_checked;  // bases needing their element count (gather/scatter)
     // Instructions a skeleton implements itself (collapsed reductions and their identities).
@@ -499,7 +500,26 @@ void Writer::collect() {
     }
     std::sort(const_instrs.begin(), const_instrs.end(),
               [](const InstrPtr &a, const InstrPtr &b) { return a->origin_id < b->origin_id; });
-    for (size_t j = 0; j < const_instrs.size(); ++j) const_of_origin[const_instrs[j]->origin_id] = static_cast<int>(j);
+    // Constants of equal type and value share one parameter (a loop body repeats 1.0, 0.5, its
+    // coefficients ...; each parameter takes SGPRs, and past the SGPR budget the compiler spills them
+    // to VGPR lanes and reads them back in every iteration).  The flush key records which constants
+    // of a flush are equal, so a cached plan is reused only where the same ones are.
+    for (size_t j = 0; j < const_instrs.size(); ++j) {
+        const bh_constant &c = const_instrs[j]->constant;
+        int id = -1;
+        if (c.type != bh_type::R123) {
+            for (size_t k = 0; k < const_params.size() && id < 0; ++k) {
+                const bh_constant &d = const_params[k]->constant;
+                if (d.type == c.type && std::memcmp(&d.value, &c.value, bh_type_size(c.type)) == 0)
+                    id = static_cast<int>(k);
+            }
+        }
+        if (id < 0) {
+            id = static_cast<int>(const_params.size());
+            const_params.push_back(const_instrs[j]);
+        }
+        const_of_origin[const_instrs[j]->origin_id] = id;
+    }
     temps = unit.getAllTemps();
     std::set<bh_base *> non_temps = unit.getAllNonTemps();
     for (const bh_base *b : always_array) non_temps.insert(const_cast<bh_base *>(b));
@@ -715,21 +735,21 @@ std::vector<KernelParam> Writer::parameters() {
         const int64_t n = b->nelem();
         lit("int64_t", "nb" + std::to_string(bid.at(b)), &n, 8, 8);
     }
-    for (size_t j = 0; j < const_instrs.size(); ++j) {
-        const bh_constant &c = const_instrs[j]->constant;
+    for (size_t j = 0; j < const_params.size(); ++j) {
+        const bh_constant &c = const_params[j]->constant;
         const std::string cn = "c" + std::to_string(j);
         if (c.type == bh_type::R123) {
             lit("uint64_t", cn + "_start", &c.value.r123.start, 8, 8);
-            kp.args.back().const_origin = const_instrs[j]->origin_id;
+            kp.args.back().const_origin = const_params[j]->origin_id;
             kp.args.back().const_part = 0;
             lit("uint64_t", cn + "_key", &c.value.r123.key, 8, 8);
-            kp.args.back().const_origin = const_instrs[j]->origin_id;
+            kp.args.back().const_origin = const_params[j]->origin_id;
             kp.args.back().const_part = 1;
         } else {
             const int size = bh_type_size(c.type);
             const int align = bh_type_is_complex(c.type) ? size / 2 : size;
             lit(hip_type(c.type), cn, &c.value, size, align);
-            kp.args.back().const_origin = const_instrs[j]->origin_id;
+            kp.args.back().const_origin = const_params[j]->origin_id;
         }
     }
     if (kp.uses_index_check) {

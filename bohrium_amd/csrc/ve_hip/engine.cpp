@@ -418,6 +418,10 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
     std::string key;
     key.reserve(instrs.size() * 96);
     auto put = [&](int64_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    // Equal constants share a kernel parameter: the key holds, for each constant, the position of the
+    // first constant of equal type and value in the flush.
+    std::unordered_map<std::string, int64_t> first_equal;
+    int64_t nconst = 0;
     for (const bh_instruction *i : instrs) {
         put(i->opcode);
         put(i->constructor ? 1 : 0);
@@ -428,6 +432,12 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
                 put(static_cast<int64_t>(i->constant.type));
                 // A sweep's constant is its axis, which shapes the loop nest.
                 if (bh_opcode_is_sweep(i->opcode)) put(i->constant.get_int64());
+                if (i->constant.type != bh_type::R123) {
+                    std::string v(reinterpret_cast<const char *>(&i->constant.value), bh_type_size(i->constant.type));
+                    v.push_back(static_cast<char>(i->constant.type));
+                    put(first_equal.emplace(std::move(v), nconst).first->second);
+                }
+                ++nconst;
                 continue;
             }
             auto it = base2ord.find(v.base);
