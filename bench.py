@@ -76,6 +76,29 @@ def launches(bh):
     return [l.split() for l in bh.message("launches").splitlines() if l.strip()]
 
 
+def mc_pi_secondary(bh, workloads):
+    """Monte-Carlo pi at N = 1e7: philox uniforms fused with the test and the int64 count (one
+    FULL_REDUCE launch and its finalize); compute-bound, reported as points per second."""
+    n, reps = 10_000_000, 20
+    count = int(workloads.mc_pi(n).asnumpy())
+    bh.message("profile: on")
+    launches(bh)
+    for _ in range(3):
+        workloads.mc_pi(n).asnumpy()
+    ls = launches(bh)
+    bh.message("profile: off")
+    red = [float(l[3]) for l in ls if l[0] == "reduce" and int(l[1]) == n]
+    kernel_ms = sum(red) / max(1, len(red))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        workloads.mc_pi(n).asnumpy()
+    wall = (time.perf_counter() - t0) / reps
+    return {"workload": "Monte-Carlo pi, N=1e7 points from philox2x32-10 (key 42), int64 count",
+            "count": count, "pi_estimate": 4.0 * count / n, "kernel_ms": round(kernel_ms, 4),
+            "points_per_s_kernel": round(n / (kernel_ms * 1e-3), 1) if kernel_ms > 0 else None,
+            "ms_per_evaluation_wall": round(wall * 1e3, 4)}
+
+
 def jacobi_secondary(bh, workloads):
     """2-D Jacobi fp32 8192^2, 200 sweeps replayed as one graph per sweep."""
     n, iters = 8192, 200
@@ -219,7 +242,7 @@ def main():
     secondary = None
     if not args.no_secondary:
         del S, K, T
-        secondary = {"jacobi": jacobi_secondary(bh, workloads), "batched": batched_secondary(bh, workloads, api),
+        secondary = {"mc_pi": mc_pi_secondary(bh, workloads), "jacobi": jacobi_secondary(bh, workloads), "batched": batched_secondary(bh, workloads, api),
                      "blas_gemm": gemm_secondary(bh)}
 
     cpu = None

@@ -417,6 +417,18 @@ private:
         }
         return s.str();
     }
+    // Strides are part of the plan key, so views with equal strides share one parameter (and the
+    // compiler one product i * stride for all of them, e.g. the five loads of a stencil).
+    std::map<int64_t, int> stride_ids;
+    std::vector<int64_t> stride_vals;
+    int stride_id(int64_t stride) {
+        auto it = stride_ids.find(stride);
+        if (it != stride_ids.end()) return it->second;
+        const int id = static_cast<int>(stride_vals.size());
+        stride_ids[stride] = id;
+        stride_vals.push_back(stride);
+        return id;
+    }
     std::string index(const bh_view &v, int hidden_axis = BH_MAXDIM, int off_axis = -1, int off = 0) {
         std::ostringstream s;
         const int k = os_id(v);
@@ -430,7 +442,7 @@ private:
                     s << " + (i" << t << " + (i" << t << " == 0 ? 0 : " << off << "))";
                 else
                     s << " + i" << t;
-                if (v.stride[d] != 1) s << " * vs" << k << "_" << d;
+                if (v.stride[d] != 1) s << " * vs" << stride_id(v.stride[d]);
             }
         }
         return s.str();
@@ -727,9 +739,8 @@ std::vector<KernelParam> Writer::parameters() {
         } else if (v.hasSlide()) {
             kp.reusable = false;
         }
-        for (int64_t d = 0; d < v.ndim; ++d)
-            lit("int64_t", "vs" + std::to_string(k) + "_" + std::to_string(d), &v.stride[d], 8, 8);
     }
+    for (size_t j = 0; j < stride_vals.size(); ++j) lit("int64_t", "vs" + std::to_string(j), &stride_vals[j], 8, 8);
     for (size_t j = 0; j < loops.size(); ++j) lit("int64_t", "n" + std::to_string(j), &loops[j]->size, 8, 8);
     for (const bh_base *b : index_checked) {
         const int64_t n = b->nelem();
