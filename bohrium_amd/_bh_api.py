@@ -151,10 +151,16 @@ def trace_stop():
 
 
 def _bid(handle):
+    """The trace id of a live base.  Ids are never reused; a handle (the runtime's base address)
+    can be, once its base is freed, and then names a new base with a new id (see _retire)."""
     if handle not in _base_ids:
         _base_ids[handle] = _next_base_id[0]
         _next_base_id[0] += 1
     return _base_ids[handle]
+
+
+def _retire(handle):
+    _base_ids.pop(handle, None)
 
 
 def _rec(*record):
@@ -259,6 +265,7 @@ def free(handle):
         _fake_handles.discard(handle)
         if _trace is not None:
             _rec("free", _bid(handle))
+        _retire(handle)
         return
     if not _initialised[0]:
         return
@@ -271,6 +278,7 @@ def free(handle):
         _check(_lib.bhc_free(handle))
     if _trace is not None:
         _rec("free", _bid(handle))
+    _retire(handle)
 
 
 def sync(handle):
