@@ -93,7 +93,7 @@ struct WriterConfig {
     int reduce_max_grid = 4096;  // workgroups for FULL_REDUCE
     int reduce_min_items = 4;    // FULL_REDUCE: elements per thread before another workgroup is added
     int scan_items = 8;      // elements per thread per scan tile
-    int map_unroll = 2;      // grid-stride elements per thread issued together (MAP)
+    int map_unroll = 1;      // grid-stride elements per thread issued together (MAP)
     bool row_reduce = true;  // run row reductions with one wave per row (ROW_REDUCE)
     int waves_per_eu = 0;    // >0: amdgpu_waves_per_eu occupancy hint on the main kernels
     int reduce_unroll = 1;   // the same for FULL_REDUCE

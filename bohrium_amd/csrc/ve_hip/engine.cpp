@@ -127,7 +127,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     wcfg.resident_grid = cfg.defaultGet<bool>("resident_grid", false);
     wcfg.reduce_min_items = std::max(1, cfg.defaultGet<int>("reduce_min_items", 4));
     wcfg.scan_items = cfg.defaultGet<int>("scan_items", 8);
-    wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 2));
+    wcfg.map_unroll = std::max(1, cfg.defaultGet<int>("map_unroll", 1));
     wcfg.reduce_unroll = std::max(1, cfg.defaultGet<int>("reduce_unroll", 1));
     wcfg.row_reduce = cfg.defaultGet<bool>("row_reduce", true);
     wcfg.waves_per_eu = std::max(0, cfg.defaultGet<int>("waves_per_eu", 0));
