@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, default=20_000_000)
     p.add_argument("--no-secondary", action="store_true")
+    p.add_argument("--gemm", action="store_true",
+                   help="also time blas_gemm on 8192^2 operands (python -m bohrium_amd.blas does the same)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 256 MiB device copy 5 times (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
@@ -242,8 +244,10 @@ def main():
     secondary = None
     if not args.no_secondary:
         del S, K, T
-        secondary = {"mc_pi": mc_pi_secondary(bh, workloads), "jacobi": jacobi_secondary(bh, workloads), "batched": batched_secondary(bh, workloads, api),
-                     "blas_gemm": gemm_secondary(bh)}
+        secondary = {"mc_pi": mc_pi_secondary(bh, workloads), "jacobi": jacobi_secondary(bh, workloads),
+                     "batched": batched_secondary(bh, workloads, api)}
+        if args.gemm:
+            secondary["blas_gemm"] = gemm_secondary(bh)
 
     cpu = None
     if not args.no_cpu:
