@@ -798,6 +798,9 @@ std::vector<const bh_view *> Writer::preloadable(const LoopB &l) {
 // `unroll` elements (strided by the grid) first load every read-only input of all of them
 // into registers and then compute them, so a wave has all of its loads in flight at once;
 // the remaining elements run one at a time.
+// Full reductions with at most this many instructions in their loop count as light (half the grid).
+constexpr size_t kLightReduceInstrs = 16;
+
 void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
     body << "  const int64_t st_ = (int64_t)bh_nblk * BH_WG;\n";
     body << "  int64_t g_ = (int64_t)bh_bid * BH_WG + threadIdx.x;\n";
@@ -1149,8 +1152,14 @@ void Writer::emit_full_reduce() {
     kp.work_items = total;
     // Each thread folds at least reduce_min_items elements before another workgroup (and its
     // partial) is added; at most reduce_max_grid workgroups.
+    // A light loop body (a few operations per element, a stencil) streams better from half as many
+    // workgroups, each folding twice the elements; a heavy one (Black-Scholes' ~70 operations) from
+    // the full number (MI355X: Jacobi 8190^2 stencil+max 5 % faster at 2048 than 4096 workgroups,
+    // Black-Scholes 2-4 % slower).
     const uint64_t per_wg = static_cast<uint64_t>(kp.block) * static_cast<uint64_t>(std::max(1, cfg.reduce_min_items));
-    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((total + per_wg - 1) / per_wg, cfg.reduce_max_grid));
+    const bool light = jitk::allInstr(l0).size() <= kLightReduceInstrs;
+    const uint64_t cap = std::max<uint64_t>(1, light ? cfg.reduce_max_grid / 2 : cfg.reduce_max_grid);
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((total + per_wg - 1) / per_wg, cap));
     kp.grid = grid;
 
     // Kernel scope: every rank-0 reduction output is a per-thread accumulator that starts at the identity.
