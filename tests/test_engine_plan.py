@@ -106,3 +106,34 @@ def test_fusers_keep_random_programs_parallel(plans):
     for name in ("dag_greedy", "dag_serial"):
         assert plans[name], name
         assert "serial" not in plans[name], (name, plans[name])
+
+
+GRID_DRIVER = r"""
+import sys
+sys.path.insert(0, %(root)r)
+import bohrium_amd as bh
+from bohrium_amd import workloads
+def reduce_grids(fn):
+    fn()
+    bh.flush()
+    return [int(l.split()[1]) for l in bh.message("plan log").splitlines() if l.startswith("reduce")]
+G = workloads.jacobi_grid(8192)
+bh.flush(); bh.message("plan log")
+print("JACOBI", reduce_grids(lambda: workloads.jacobi_step(G)))
+S, K, T = workloads.black_scholes_inputs(20_000_000)
+bh.flush(); bh.message("plan log")
+print("BS", reduce_grids(lambda: workloads.black_scholes_step(S, K, T, 0)))
+"""
+
+
+def test_light_reductions_take_half_the_workgroup_cap():
+    """Full reductions with a light loop body (the Jacobi stencil + max|delta|) use half of
+    reduce_max_grid (4096) workgroups, heavy ones (a Black-Scholes step) the whole cap; measured on
+    MI355X, each is the faster choice for its kind."""
+    env = dict(os.environ, BH_HIP_DRY_RUN="1")
+    r = subprocess.run([sys.executable, "-c", GRID_DRIVER % {"root": ROOT}], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:]
+    out = dict(l.split(" ", 1) for l in r.stdout.splitlines() if l.startswith(("JACOBI", "BS")))
+    assert json.loads(out["JACOBI"]) == [2048]
+    assert json.loads(out["BS"]) == [4096]
