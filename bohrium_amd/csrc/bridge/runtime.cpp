@@ -38,20 +38,52 @@ struct Runtime {
         stack = ComponentFace{top, 0};
     }
 
-    // Auto-flush happens before queueing, so the instruction queued last is always pending.
+    // Auto-flush happens before queueing, so the instruction queued last is always pending
+    // (bhc_op_slide amends it).  A base enters `freed` only after its BH_FREE is queued, so the
+    // auto-flush here never deletes a base whose BH_FREE has not been executed yet (the reference
+    // queues before it flushes for the same reason, bridge/cxx/src/Runtime.cpp:33-42).
     void enqueue(bh_instruction &&instr) {
         if (threshold > 0 && static_cast<int64_t>(queue.size()) >= threshold) flush(1, nullptr);
         queue.push_back(std::move(instr));
     }
 
     void flush(uint64_t nrepeats, bh_base *cond) {
+        // Every base in `freed` has its BH_FREE in this batch.
+        std::vector<bh_base *> batch_freed;
+        batch_freed.swap(freed);
         BhIR bhir(std::move(queue), syncs, nrepeats, cond);
         queue.clear();
         syncs.clear();
-        stack.execute(&bhir);
-        for (bh_base *b : freed) delete b;
-        freed.clear();
+        try {
+            stack.execute(&bhir);
+        } catch (...) {
+            release_after_failure(batch_freed);
+            throw;
+        }
+        for (bh_base *b : batch_freed) delete b;
     }
+
+    // A failed flush may have stopped before the engine released the bases freed in it.  Hand
+    // their BH_FREEs to the stack once more, on their own; a base is deleted only when that
+    // succeeds, and leaked otherwise: deleting it unreleased would let a new base at the same
+    // address inherit its device buffer.
+    void release_after_failure(std::vector<bh_base *> &bases) {
+        if (bases.empty()) return;
+        std::vector<bh_instruction> frees;
+        for (bh_base *b : bases) frees.emplace_back(BH_FREE, std::vector<bh_view>{bh_view(b)});
+        BhIR bhir(std::move(frees), std::set<bh_base *>{}, 1, nullptr);
+        try {
+            stack.execute(&bhir);
+        } catch (...) {
+            ++leaked_after_failure;
+            bases.clear();
+            return;
+        }
+        for (bh_base *b : bases) delete b;
+        bases.clear();
+    }
+
+    uint64_t leaked_after_failure = 0;
 };
 
 Runtime *g_rt = nullptr;
@@ -196,9 +228,10 @@ int bhc_free(bhc_base_t base) {
     return guarded([&] {
         bh_base *b = static_cast<bh_base *>(base);
         bh_instruction instr(BH_FREE, {bh_view(b)});
-        rt().freed.push_back(b);
-        rt().syncs.erase(b);
-        rt().enqueue(std::move(instr));
+        Runtime &r = rt();
+        r.syncs.erase(b);
+        r.enqueue(std::move(instr));  // may flush the instructions queued before this one
+        r.freed.push_back(b);         // deleted after the flush that executes this BH_FREE
     });
 }
 

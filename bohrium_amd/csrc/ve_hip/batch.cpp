@@ -280,6 +280,7 @@ void EngineHip::launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh
     } a = {g.dev, reinterpret_cast<const uint32_t *>(g.dev + table_off), static_cast<uint32_t>(k),
            static_cast<uint32_t>(stride)};
     check(bhhip_launch(g.fn, blocks, p0.block, &a, sizeof(a)), "batched kernel launch");
+    errors_unchecked |= p0.uses_error_flag;
     if (recording != nullptr)
         recording->push_back({g.fn, blocks, p0.block,
                               {reinterpret_cast<uint8_t *>(&a), reinterpret_cast<uint8_t *>(&a) + sizeof(a)}});

@@ -681,6 +681,7 @@ void Writer::emit_instr(const bh_instruction &instr, Scope &sc, int ind) {
              << "if (g_ < 0 || g_ >= nb" << bid.at(in.base) << ") { atomicOr(bh_err, BH_ERR_INDEX); } else { "
              << out << " = a" << bid.at(in.base) << "[g_]; } }\n";
         kp.uses_index_check = true;
+        kp.uses_error_flag = true;
         return;
     } else if (instr.opcode == BH_SCATTER || instr.opcode == BH_COND_SCATTER) {
         const bh_view &out = instr.operand[0];
@@ -690,6 +691,7 @@ void Writer::emit_instr(const bh_instruction &instr, Scope &sc, int ind) {
         body << "{ if (g_ < 0 || g_ >= nb" << bid.at(out.base) << ") { atomicOr(bh_err, BH_ERR_INDEX); } else { a"
              << bid.at(out.base) << "[g_] = " << ref(sc, instr.operand[1]) << "; } } }\n";
         kp.uses_index_check = true;
+        kp.uses_error_flag = true;
         return;
     } else if (bh_opcode_is_accumulate(instr.opcode)) {
         ops.push_back(ref(sc, instr.operand[0]));
@@ -763,7 +765,7 @@ std::vector<KernelParam> Writer::parameters() {
             kp.args.back().const_origin = const_params[j]->origin_id;
         }
     }
-    if (kp.uses_index_check) {
+    if (kp.uses_error_flag) {
         params.push_back({"unsigned int *", "bh_err", true});
         ArgSpec a;
         a.kind = ArgSpec::ERRFLAG_PTR;
@@ -1326,6 +1328,7 @@ void Writer::emit_scan() {
     const std::string ident = literal(jitk::sweep_identity(acc->opcode, t));
     scan_instr = acc.get();
     scan_sink = "x_";
+    kp.uses_error_flag = true;
 
     body << "  auto op_ = [](" << T << " x, " << T << " y) -> " << T << " { return "
          << combine_expr(acc->opcode, "x", "y") << "; };\n";
@@ -1381,7 +1384,14 @@ void Writer::emit_scan() {
     body << "        const int64_t j_ = hi_ - lane_;\n";
     body << "        " << T << " w_ = " << ident << ";\n";
     body << "        unsigned int st_ = 2u;\n";
-    body << "        if (j_ >= 0) { while ((st_ = " << LB << "::get(bh_rec, j_, fb_, &w_)) == 0u) {} }\n";
+    // A record that never arrives (lost or of another epoch) must not hang the device: after
+    // BH_SCAN_SPIN polls the wait gives up, raises BH_ERR_SCAN and the host reports it.
+    body << "        if (j_ >= 0) {\n";
+    body << "          unsigned int sp_ = 0u;\n";
+    body << "          while ((st_ = " << LB << "::get(bh_rec, j_, fb_, &w_)) == 0u) {\n";
+    body << "            if (++sp_ >= BH_SCAN_SPIN) { atomicOr(bh_err, BH_ERR_SCAN); st_ = 2u; break; }\n";
+    body << "          }\n";
+    body << "        }\n";
     body << "        const unsigned long long pm_ = __ballot(st_ == 2u);\n";
     body << "        const int first_ = pm_ ? __ffsll((long long)pm_) - 1 : 64;\n";
     body << "        ex_ = op_(bh_wave_reduce(lane_ <= first_ ? w_ : " << ident << ", op_), ex_);\n";

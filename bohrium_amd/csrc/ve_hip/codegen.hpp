@@ -74,7 +74,8 @@ struct KernelPlan {
     std::set<bh_base *> written;
     /* Scratch bytes (partials / look-back state) this launch needs. */
     int64_t scratch_bytes = 0;
-    bool uses_index_check = false;
+    bool uses_index_check = false;  // gather/scatter: the flush checks the error flag when it ends
+    bool uses_error_flag = false;   // takes bh_err (index checks, bounded look-back waits)
     uint64_t work_items = 0;  // elements of the iteration space (for statistics / cost model features)
     /* Filled when the plan is cached: ordinals of params/written bases and the compiled function. */
     std::vector<int> param_ordinals, written_ordinals;

@@ -207,6 +207,9 @@ void EngineHip::copyToHost(bh_base *base) {
     stats.d2h_bytes += b.bytes;
     stats.time_copy += seconds_since(t);
     b.host_valid = true;
+    // Kernels that raise errors without a check at the end of their flush (scans) are checked at
+    // the next copy to the host, which has waited for them anyway.
+    if (errors_unchecked) checkErrors();
 }
 
 void EngineHip::copyAllToHost() {
@@ -215,6 +218,7 @@ void EngineHip::copyAllToHost() {
 
 void EngineHip::release(bh_base *base) {
     released_now.insert(base);
+    ++releases;
     auto it = bufs.find(base);
     if (it != bufs.end()) {
         if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
@@ -276,6 +280,7 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
     }
     check(bhhip_launch(plan.function, plan.grid, plan.block, buf, n), "kernel launch");
     ++stats.launches;
+    errors_unchecked |= plan.uses_error_flag;
     if (recording != nullptr) recording->push_back({plan.function, plan.grid, plan.block, {buf, buf + n}});
     if (plan.fin_function != nullptr) {
         check(bhhip_launch(plan.fin_function, 1, plan.block, buf, n), "kernel launch");
@@ -523,12 +528,15 @@ void EngineHip::graphLaunch(void *graph, size_t nlaunches) {
 
 void EngineHip::checkErrors() {
     if (dry_run) return;
+    errors_unchecked = false;
     unsigned int flag = 0;
     check(bhhip_memcpy_d2h(&flag, counters + 1, sizeof(flag)), "reading the error flag");
     if (flag != 0) {
         const unsigned int zero = 0;
         check(bhhip_memcpy_h2d(counters + 1, &zero, sizeof(zero)), "clearing the error flag");
-        throw std::out_of_range("hip engine: gather/scatter index out of bounds");
+        if (flag & 1u) throw std::out_of_range("hip engine: gather/scatter index out of bounds");
+        throw std::runtime_error("hip engine: a scan tile gave up waiting for the prefix of an earlier tile "
+                                 "(its accumulate results are wrong)");
     }
 }
 
@@ -829,6 +837,7 @@ std::string EngineHip::statistics() const {
     for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
     ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
     ss << "missed_frees: " << missed_frees << "\n";
+    ss << "buffers: " << bufs.size() << "\nreleases: " << releases << "\n";
     ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
     ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";

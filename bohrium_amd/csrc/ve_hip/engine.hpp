@@ -211,7 +211,9 @@ private:
     std::unordered_map<std::string, CachedFlush> flush_cache;
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
     uint64_t missed_frees = 0;
+    uint64_t releases = 0;  // bases released (BH_FREE handled) since the engine started
     bool last_index_checks = false;
+    bool errors_unchecked = false;  // a launch since the last error-flag check can raise an error
     uint64_t scratch_allocs = 0;
     std::map<std::string, void *> graph_cache;
     std::string flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
