@@ -28,7 +28,7 @@ HIPFLAGS = ["-std=c++17", "-O2", "-fPIC", "-I" + INC, "--offload-arch=gfx950",
 
 CORE = ["core/bh_type.cpp", "core/bh_constant.cpp", "core/bh_view.cpp", "core/bh_instruction.cpp",
         "core/bh_opcode.cpp", "core/bh_config_parser.cpp", "core/bh_component.cpp",
-        "core/bh_main_memory.cpp", "jitk/block.cpp", "jitk/fuser.cpp"]
+        "core/bh_main_memory.cpp", "core/bh_mem_signal.cpp", "jitk/block.cpp", "jitk/fuser.cpp"]
 BRIDGE = ["bridge/runtime.cpp"]
 NODE = ["vem_node/node.cpp"]
 VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp", "ve_hip/features.cpp"]

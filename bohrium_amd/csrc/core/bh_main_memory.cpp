@@ -6,6 +6,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 
 namespace {
 
@@ -16,6 +17,8 @@ struct Entry {
 
 // Most-recently-freed first; an exact-size match is reused.
 std::list<Entry> g_cache;
+// Live allocations (address -> bytes): lets a caller check that a page range is this allocator's alone.
+std::unordered_map<void *, uint64_t> g_live;
 uint64_t g_cached_bytes = 0;
 uint64_t g_limit = 1ull << 30;
 std::mutex g_mu;
@@ -47,16 +50,27 @@ void *bh_memory_malloc(uint64_t nbytes) {
                 void *m = it->mem;
                 g_cached_bytes -= it->nbytes;
                 g_cache.erase(it);
+                g_live[m] = nbytes;
                 return m;
             }
         }
     }
-    return raw_alloc(nbytes);
+    void *m = raw_alloc(nbytes);
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_live[m] = nbytes;
+    return m;
+}
+
+bool bh_memory_owns(const void *mem, uint64_t nbytes) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_live.find(const_cast<void *>(mem));
+    return it != g_live.end() && it->second >= nbytes;
 }
 
 void bh_memory_free(void *mem, uint64_t nbytes) {
     if (mem == nullptr) return;
     std::lock_guard<std::mutex> lock(g_mu);
+    g_live.erase(mem);
     if (nbytes > g_limit) {
         raw_free(mem, nbytes);
         return;

@@ -18,5 +18,8 @@ void bh_data_free(bh_base *base);
 /* Raw page-aligned allocation through the same cache. */
 void *bh_memory_malloc(uint64_t nbytes);
 void bh_memory_free(void *mem, uint64_t nbytes);
+/* True when `mem` is a live bh_memory_malloc allocation of at least `nbytes` bytes: its pages
+ * (a mapping of their own, page aligned) belong to it alone. */
+bool bh_memory_owns(const void *mem, uint64_t nbytes);
 /* Limit (bytes) of memory kept in the reuse cache. */
 void bh_set_malloc_cache_limit(uint64_t nbytes);

@@ -1,6 +1,10 @@
 #include "engine.hpp"
 
+#include <sys/mman.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdlib>
 #include <cstdio>
@@ -12,6 +16,7 @@
 #include <bohrium/bh_main_memory.hpp>
 #include "../../../include/bhhip.h"
 #include "../../../include/bh_ve_hip.h"
+#include "../../../include/bh_mem_signal.h"
 
 namespace bohrium {
 namespace vehip {
@@ -38,6 +43,25 @@ uint64_t fnv1a(const std::string &s) {
         h *= 1099511628211ull;
     }
     return h;
+}
+
+// Host writes into guarded pages seen by the fault handler (all engines of the process).
+std::atomic<uint64_t> g_host_write_faults{0};
+
+// Fault handler of a guarded host copy (signal context, the faulting thread): make the pages
+// writable again and, when the host copy was current, make it the newest (the device copy is
+// uploaded again before a kernel reads it).  A write into a stale host copy leaves the device
+// copy the newest: such writes are lost, as they are when the reference engines hold the array.
+int on_host_write(void *, void *idx) {
+    DeviceBuffer *d = static_cast<DeviceBuffer *>(idx);
+    void *addr = d->guard_addr;
+    if (addr == nullptr || mprotect(addr, d->guard_bytes, PROT_READ | PROT_WRITE) != 0) return 0;
+    d->guard_addr = nullptr;
+    d->guard_bytes = 0;
+    bh_mem_signal_detach(addr);
+    if (d->host_valid) d->dev_valid = false;
+    g_host_write_faults.fetch_add(1);
+    return 1;
 }
 
 std::string default_cache_dir() {
@@ -133,6 +157,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     wcfg.waves_per_eu = std::max(0, cfg.defaultGet<int>("waves_per_eu", 0));
     unflatten = cfg.defaultGet<bool>("unflatten_reductions", true);
     resident = cfg.defaultGet<bool>("resident", true);
+    track_host_writes = cfg.defaultGet<bool>("host_write_tracking", true);
     graph_repeats = cfg.defaultGet<bool>("graph_repeats", true);
     batch_min = cfg.defaultGet<int>("batch_min", 8);
     profile = cfg.defaultGet<bool>("prof", false);
@@ -148,6 +173,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
 }
 
 EngineHip::~EngineHip() {
+    for (auto &kv : bufs) unguardHost(kv.second);
     if (dry_run) return;
     bhhip_synchronize();
     for (auto &kv : graph_cache) bhhip_graph_destroy(kv.second);
@@ -202,6 +228,7 @@ void EngineHip::copyToHost(bh_base *base) {
     }
     if (!b.dev_valid || b.host_valid || b.ptr == nullptr) return;
     const auto t = std::chrono::steady_clock::now();
+    unguardHost(b);
     bh_data_malloc(base);
     check(bhhip_memcpy_d2h(base->getDataPtr(), b.ptr, b.bytes), "copy to host");
     stats.d2h_bytes += b.bytes;
@@ -213,17 +240,76 @@ void EngineHip::copyToHost(bh_base *base) {
 }
 
 void EngineHip::copyAllToHost() {
-    for (auto &kv : bufs) copyToHost(const_cast<bh_base *>(kv.first));
+    // The host copies become the newest: another component works on them from here on.
+    for (auto &kv : bufs) {
+        copyToHost(const_cast<bh_base *>(kv.first));
+        unguardHost(kv.second);
+        if (kv.second.host_valid) kv.second.dev_valid = false;
+    }
+}
+
+void EngineHip::guardHost(bh_base *base, DeviceBuffer &d) {
+    if (d.guard_addr != nullptr) return;
+    void *p = base->getDataPtr();
+    const uint64_t n = static_cast<uint64_t>(base->nbytes());
+    const long page = sysconf(_SC_PAGESIZE);
+    // Only pages that hold nothing but this base's data: a live allocation of the engine's allocator
+    // (a mapping of its own).  Anything else is handed out as the newest copy instead.
+    if (p == nullptr || n == 0 || page <= 0 || !bh_memory_owns(p, n)) {
+        if (d.host_valid) d.dev_valid = false;
+        return;
+    }
+    const uint64_t len = (n + page - 1) / page * page;
+    if (mprotect(p, len, PROT_READ) != 0) {
+        if (d.host_valid) d.dev_valid = false;
+        return;
+    }
+    d.guard_addr = p;
+    d.guard_bytes = len;
+    if (bh_mem_signal_attach(&d, p, len, on_host_write) != 0) {
+        mprotect(p, len, PROT_READ | PROT_WRITE);
+        d.guard_addr = nullptr;
+        d.guard_bytes = 0;
+        if (d.host_valid) d.dev_valid = false;
+        return;
+    }
+    ++host_guards;
+}
+
+void EngineHip::unguardHost(DeviceBuffer &d) {
+    void *p = d.guard_addr;
+    if (p == nullptr) return;
+    bh_mem_signal_detach(p);
+    if (mprotect(p, d.guard_bytes, PROT_READ | PROT_WRITE) != 0)
+        throw std::runtime_error("hip engine: cannot make a guarded host copy writable again");
+    d.guard_addr = nullptr;
+    d.guard_bytes = 0;
+}
+
+void EngineHip::eraseBuffer(std::unordered_map<const bh_base *, DeviceBuffer>::iterator it) {
+    unguardHost(it->second);
+    if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
+    bufs.erase(it);
+}
+
+void EngineHip::handOutHost(bh_base *base) {
+    auto it = bufs.find(base);
+    if (it == bufs.end() || dry_run) return;
+    DeviceBuffer &d = it->second;
+    if (!d.host_valid || base->getDataPtr() == nullptr) return;
+    if (!d.dev_valid || d.ptr == nullptr) return;  // the host copy is the only current one anyway
+    if (track_host_writes) {
+        guardHost(base, d);
+    } else {
+        d.dev_valid = false;
+    }
 }
 
 void EngineHip::release(bh_base *base) {
     released_now.insert(base);
     ++releases;
     auto it = bufs.find(base);
-    if (it != bufs.end()) {
-        if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
-        bufs.erase(it);
-    }
+    if (it != bufs.end()) eraseBuffer(it);
     bh_data_free(base);
 }
 
@@ -632,10 +718,9 @@ void EngineHip::handleExecution(BhIR *bhir) {
         copyToHost(b);
         if (!resident) {
             auto it = bufs.find(b);
-            if (it != bufs.end()) {
-                if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
-                bufs.erase(it);
-            }
+            if (it != bufs.end()) eraseBuffer(it);
+        } else {
+            handOutHost(b);
         }
     }
     stats.time_total += seconds_since(t_total);
@@ -694,13 +779,18 @@ void *EngineHip::getMemoryPointer(bh_base &base, bool copy2host, bool force_allo
     void *ret = b->getDataPtr();
     auto it = bufs.find(b);
     if (!resident && it != bufs.end()) {
-        if (!it->second.external) pool.release(it->second.ptr, it->second.bytes);
-        bufs.erase(it);
+        eraseBuffer(it);
         it = bufs.end();
     }
     if (nullify) {
+        // The caller takes the host buffer over: no guard may stay on its pages.
+        if (it != bufs.end()) {
+            unguardHost(it->second);
+            it->second.host_valid = false;
+        }
         b->resetDataPtr(nullptr);
-        if (it != bufs.end()) it->second.host_valid = false;
+    } else if (it != bufs.end()) {
+        handOutHost(b);
     }
     return ret;
 }
@@ -786,6 +876,7 @@ std::string EngineHip::userKernel(const std::string &kernel, std::vector<bh_view
 void EngineHip::markHostWritten(bh_base *base) {
     auto it = bufs.find(base);
     if (it == bufs.end() || base->getDataPtr() == nullptr) return;
+    unguardHost(it->second);
     it->second.host_valid = true;
     it->second.dev_valid = false;
 }
@@ -799,6 +890,7 @@ void EngineHip::markDeviceWritten(bh_base *base) {
 void EngineHip::setMemoryPointer(bh_base *base, bool host_ptr, void *mem) {
     DeviceBuffer &d = buffer(base);
     if (host_ptr) {
+        unguardHost(d);
         if (base->getDataPtr() != nullptr && base->getDataPtr() != mem) bh_data_free(base);
         base->resetDataPtr(mem);
         d.host_valid = mem != nullptr;
@@ -838,6 +930,7 @@ std::string EngineHip::statistics() const {
     ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
     ss << "missed_frees: " << missed_frees << "\n";
     ss << "buffers: " << bufs.size() << "\nreleases: " << releases << "\n";
+    ss << "host_guards: " << host_guards << "\nhost_write_faults: " << g_host_write_faults.load() << "\n";
     ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
     ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";
@@ -1043,6 +1136,10 @@ public:
             engine.setBatchMin(std::stoi(msg.substr(10)));
         } else if (msg == "plans: drop") {
             engine.dropPlans();
+        } else if (msg == "host_write_tracking: on") {
+            engine.track_host_writes = true;
+        } else if (msg == "host_write_tracking: off") {
+            engine.track_host_writes = false;
         } else if (msg == "graph: on") {
             engine.graph_repeats = true;
         } else if (msg == "graph: off") {

@@ -32,6 +32,10 @@ struct DeviceBuffer {
     bool dev_valid = false;   // device holds the newest data
     bool host_valid = false;  // host buffer holds the newest data
     bool external = false;    // device memory supplied by the caller (not pooled)
+    /* Host pages made read-only after the host copy was handed out while the device copy stayed
+     * current; the first host write faults, and the fault handler makes the host copy the newest. */
+    void *guard_addr = nullptr;
+    uint64_t guard_bytes = 0;
 };
 
 /* Size-keyed reuse of device allocations (hipMalloc is far slower than a lookup). */
@@ -76,6 +80,12 @@ public:
      * (ve/opencl/engine_opencl.cpp:611-669). */
     std::string userKernel(const std::string &kernel, std::vector<bh_view> &operands, const std::string &param);
     void copyToHost(bh_base *base);
+    /* The caller receives the host copy of `base` (a sync, or getMemoryPointer with copy2host) and may
+     * write it.  With host_write_tracking the device copy stays current and the host pages become
+     * read-only until the first write; without it (or for host memory the engine cannot protect) the
+     * host copy becomes the newest at once, as in the reference GPU engines (ve/cuda/main.cpp:91-108). */
+    void handOutHost(bh_base *base);
+    bool track_host_writes = true;
     std::string info() const;
     std::string statistics() const;
     void resetStatistics() { stats = EngineStats(); }
@@ -212,6 +222,7 @@ private:
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
     uint64_t missed_frees = 0;
     uint64_t releases = 0;  // bases released (BH_FREE handled) since the engine started
+    uint64_t host_guards = 0;  // host copies handed out read-only
     bool last_index_checks = false;
     bool errors_unchecked = false;  // a launch since the last error-flag check can raise an error
     uint64_t scratch_allocs = 0;
@@ -227,6 +238,9 @@ private:
     std::vector<PendingEvent> pending_events;
 
     DeviceBuffer &buffer(bh_base *base);
+    void guardHost(bh_base *base, DeviceBuffer &d);
+    void unguardHost(DeviceBuffer &d);
+    void eraseBuffer(std::unordered_map<const bh_base *, DeviceBuffer>::iterator it);
     void ensureResident(bh_base *base);
     void release(bh_base *base);
     void ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need, bool zero);

@@ -16,12 +16,14 @@ HEADER_LIBS = {
     "bhc.h": "libbhc.so",
     "bh_ve_hip.h": "libbh_ve_hip.so",
     "bh_costmodel.h": "libbh_ve_hip.so",
+    "bh_mem_signal.h": "libbh.so",
 }
 
 
 def declared_functions(header):
     text = open(os.path.join(INCLUDE, header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"^typedef[^;]*;", "", text, flags=re.M)
     names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "while")))
 
