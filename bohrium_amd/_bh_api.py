@@ -81,6 +81,10 @@ def _load():
     lib.bhc_op_slide.argtypes = [i32, i64, i64, i64, i64, i64, i64]
     lib.bhc_base_nelem.argtypes = [vp]
     lib.bhc_base_nelem.restype = i64
+    lib.bhc_record_flushes.argtypes = [ctypes.c_char_p]
+    lib.bhc_replay_file.argtypes = [ctypes.c_char_p]
+    lib.bhc_replay_base.argtypes = [u64]
+    lib.bhc_replay_base.restype = vp
     _lib = lib
     return lib
 
@@ -351,4 +355,31 @@ def user_kernel(kernel, operands, compile_cmd, tag, param):
 
 
 def set_flush_threshold(n):
+    if _record_only[0]:
+        return  # nothing reaches the native runtime
     _check(lib().bhc_set_flush_threshold(int(n)))
+
+
+def record_flushes(path):
+    """Append every later flush to the file `path` in the BhIR wire format (None stops recording).
+    A base of a recorded flush is named in the file by its handle value (base_address)."""
+    _check(lib().bhc_record_flushes(None if path is None else os.fsencode(path)))
+
+
+def base_address(handle):
+    """The address that names `handle`'s base in recorded flushes."""
+    return int(handle)
+
+
+def replay_file(path):
+    """Execute the flushes recorded in `path` (see record_flushes) through the component stack."""
+    _check(lib().bhc_replay_file(os.fsencode(path)))
+
+
+def replay_base(address):
+    """Handle of the base that stands in for the recorded base at `address` after replay_file
+    (owned by the replay: read it, never free it)."""
+    h = lib().bhc_replay_base(ctypes.c_uint64(int(address)))
+    if not h:
+        raise BohriumError(_lib.bhc_last_error().decode(errors="replace"))
+    return h

@@ -26,12 +26,12 @@ CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wno-unused-function", "-I" 
 HIPFLAGS = ["-std=c++17", "-O2", "-fPIC", "-I" + INC, "--offload-arch=gfx950",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
-CORE = ["core/bh_type.cpp", "core/bh_constant.cpp", "core/bh_view.cpp", "core/bh_instruction.cpp",
+CORE = ["core/bh_type.cpp", "core/bh_constant.cpp", "core/bh_view.cpp", "core/bh_instruction.cpp", "core/bh_ir.cpp",
         "core/bh_opcode.cpp", "core/bh_config_parser.cpp", "core/bh_component.cpp",
         "core/bh_main_memory.cpp", "core/bh_mem_signal.cpp", "jitk/block.cpp", "jitk/fuser.cpp"]
 BRIDGE = ["bridge/runtime.cpp"]
 NODE = ["vem_node/node.cpp"]
-VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp", "ve_hip/features.cpp"]
+VE_HIP_CXX = ["ve_hip/codegen.cpp", "ve_hip/rewrite.cpp", "ve_hip/features.cpp", "ve_hip/tuner.cpp"]
 VE_HIP_HIP = ["ve_hip/bhhip.cpp", "ve_hip/engine.cpp", "ve_hip/batch.cpp", "ve_hip/costmodel.hip"]
 BLAS_HIP = ["ext_blas/blas_hip.hip"]
 

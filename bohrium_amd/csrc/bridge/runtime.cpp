@@ -4,6 +4,7 @@
  * reference bhxx runtime (bridge/cxx/src/Runtime.cpp:28-101). */
 #include "../../../include/bhc.h"
 
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -54,6 +55,7 @@ struct Runtime {
         BhIR bhir(std::move(queue), syncs, nrepeats, cond);
         queue.clear();
         syncs.clear();
+        if (recording != nullptr) record(bhir, batch_freed);
         try {
             stack.execute(&bhir);
         } catch (...) {
@@ -61,6 +63,65 @@ struct Runtime {
             throw;
         }
         for (bh_base *b : batch_freed) delete b;
+    }
+
+    // ---- recorded flushes (the BhIR wire format, core/bh_ir.cpp) ----
+    // File: "BHIRFILE", u32 version 1, then per flush: u64 archive size, the archive, and for each
+    // base the archive introduces with host data (in archive order): u64 byte count, the bytes.
+    FILE *recording = nullptr;
+    std::set<bh_base *> recorded_bases;  // bases the file has introduced and not yet seen freed
+
+    void record(BhIR &bhir, const std::vector<bh_base *> &batch_freed) {
+        std::vector<bh_base *> new_data;
+        const std::vector<char> a = bhir.writeSerializedArchive(recorded_bases, new_data);
+        const uint64_t n = a.size();
+        bool ok = std::fwrite(&n, 8, 1, recording) == 1 && std::fwrite(a.data(), 1, a.size(), recording) == a.size();
+        for (bh_base *b : new_data) {
+            const uint64_t bytes = static_cast<uint64_t>(b->nbytes());
+            ok = ok && std::fwrite(&bytes, 8, 1, recording) == 1 &&
+                 std::fwrite(b->getDataPtr(), 1, bytes, recording) == bytes;
+        }
+        ok = ok && std::fflush(recording) == 0;
+        // A freed base's address can be reused by a new base, which the file must introduce again.
+        for (bh_base *b : batch_freed) recorded_bases.erase(b);
+        if (!ok) throw std::runtime_error("bhc: writing the flush recording failed");
+    }
+
+    // Bases of replayed flushes, by the address the recording process gave them.
+    std::map<const bh_base *, bh_base> replayed;
+
+    void replay(const char *path) {
+        FILE *f = std::fopen(path, "rb");
+        if (f == nullptr) throw std::runtime_error(std::string("bhc: cannot open ") + path);
+        std::unique_ptr<FILE, int (*)(FILE *)> guard(f, std::fclose);
+        char magic[8];
+        uint32_t version = 0;
+        if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "BHIRFILE", 8) != 0 ||
+            std::fread(&version, 4, 1, f) != 1 || version != 1)
+            throw std::runtime_error(std::string("bhc: not a recorded-flush file: ") + path);
+        flush(1, nullptr);  // what this process queued comes first
+        uint64_t n = 0;
+        while (std::fread(&n, 8, 1, f) == 1) {
+            if (n > (uint64_t(1) << 34)) throw std::runtime_error("bhc: recorded flush too large");
+            std::vector<char> a(static_cast<size_t>(n));
+            if (std::fread(a.data(), 1, a.size(), f) != a.size()) throw std::runtime_error("bhc: truncated recording");
+            std::vector<bh_base *> data_recv;
+            std::set<bh_base *> frees;
+            BhIR bhir(a, replayed, data_recv, frees);
+            for (bh_base *b : data_recv) {
+                uint64_t bytes = 0;
+                if (std::fread(&bytes, 8, 1, f) != 1 || bytes != static_cast<uint64_t>(b->nbytes()))
+                    throw std::runtime_error("bhc: recorded data does not match its base");
+                void *mem = bh_memory_malloc(bytes);
+                if (std::fread(mem, 1, bytes, f) != bytes) {
+                    bh_memory_free(mem, bytes);
+                    throw std::runtime_error("bhc: truncated recording");
+                }
+                stack.setMemoryPointer(b, true, mem);
+            }
+            stack.execute(&bhir);
+            for (bh_base *b : frees) replayed.erase(b);  // released by the engine in that flush
+        }
     }
 
     // A failed flush may have stopped before the engine released the bases freed in it.  Hand
@@ -153,6 +214,7 @@ int bhc_shutdown(void) {
     return guarded([&] {
         if (g_rt == nullptr) return;
         g_rt->flush(1, nullptr);
+        if (g_rt->recording != nullptr) std::fclose(g_rt->recording);
         delete g_rt;
         g_rt = nullptr;
     });
@@ -229,6 +291,8 @@ int bhc_free(bhc_base_t base) {
         bh_base *b = static_cast<bh_base *>(base);
         bh_instruction instr(BH_FREE, {bh_view(b)});
         Runtime &r = rt();
+        for (const auto &kv : r.replayed)
+            if (&kv.second == b) throw std::invalid_argument("bhc_free: replayed bases belong to the replay");
         r.syncs.erase(b);
         r.enqueue(std::move(instr));  // may flush the instructions queued before this one
         r.freed.push_back(b);         // deleted after the flush that executes this BH_FREE
@@ -237,6 +301,42 @@ int bhc_free(bhc_base_t base) {
 
 int bhc_sync(bhc_base_t base) {
     return guarded([&] { rt().syncs.insert(static_cast<bh_base *>(base)); });
+}
+
+int bhc_record_flushes(const char *path) {
+    return guarded([&] {
+        Runtime &r = rt();
+        r.flush(1, nullptr);  // the queue so far belongs to no recording
+        if (r.recording != nullptr) {
+            std::fclose(r.recording);
+            r.recording = nullptr;
+        }
+        r.recorded_bases.clear();
+        if (path == nullptr || *path == '\0') return;
+        FILE *f = std::fopen(path, "wb");
+        if (f == nullptr) throw std::runtime_error(std::string("bhc: cannot write ") + path);
+        const uint32_t version = 1;
+        if (std::fwrite("BHIRFILE", 1, 8, f) != 8 || std::fwrite(&version, 4, 1, f) != 1) {
+            std::fclose(f);
+            throw std::runtime_error(std::string("bhc: cannot write ") + path);
+        }
+        r.recording = f;
+    });
+}
+
+int bhc_replay_file(const char *path) {
+    return guarded([&] { rt().replay(path); });
+}
+
+bhc_base_t bhc_replay_base(uint64_t recorded_address) {
+    bhc_base_t ret = nullptr;
+    guarded([&] {
+        auto &m = rt().replayed;
+        auto it = m.find(reinterpret_cast<const bh_base *>(static_cast<uintptr_t>(recorded_address)));
+        if (it == m.end()) throw std::out_of_range("bhc: no replayed base was recorded at that address");
+        ret = &it->second;
+    });
+    return ret;
 }
 
 int bhc_flush(void) {

@@ -297,11 +297,7 @@ void EngineHip::launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh
     batched_items += k;
     stats.launches_by_strategy[std::string("batch-") + strategy_name(p0.strategy)] += 1;
     for (const PlanRef &r : g.items) {
-        for (int o : cf.kernels[r.kernel].plans[r.plan].written_ordinals) {
-            DeviceBuffer &d = buffer(ord2base[o]);
-            d.dev_valid = true;
-            d.host_valid = false;
-        }
+        for (int o : cf.kernels[r.kernel].plans[r.plan].written_ordinals) noteWrite(ord2base[o], buffer(ord2base[o]));
     }
     if (profile) {
         bhhip_event_record(ev1);

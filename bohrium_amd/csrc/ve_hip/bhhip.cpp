@@ -317,6 +317,17 @@ int bhhip_event_elapsed_ms(void *start, void *stop, float *ms) {
     return 0;
 }
 
+int bhhip_event_done(void *event, int *done) {
+    const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return 0;
+    }
+    HIPCHK(e, "hipEventQuery");
+    *done = 1;
+    return 0;
+}
+
 int bhhip_event_synchronize(void *event) {
     HIPCHK(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
     return 0;

@@ -1129,9 +1129,12 @@ std::map<const bh_base *, InstrPtr> collapsible_pairs(const LoopB &l0, const std
 // thread 0.  The kernel boundary orders the two, so there is no fence, atomic ticket
 // or counter to reset, the sum order is fixed, and the pair can be graph-captured.
 void Writer::emit_full_reduce() {
-    size_t pos = 0;
-    for (size_t k = 0; k < unit._block_list.size(); ++k)
+    // The unit is [identity writers, the sweeping loop, tail blocks]; tail blocks may themselves be
+    // small loops with sweeps (a 256-element sum or scan), so the sweeping loop is the first one.
+    size_t pos = unit._block_list.size();
+    for (size_t k = 0; k < unit._block_list.size() && pos == unit._block_list.size(); ++k)
         if (!unit._block_list[k].isInstr() && !unit._block_list[k].getLoop()._sweeps.empty()) pos = k;
+    if (pos == unit._block_list.size()) throw std::logic_error("FULL_REDUCE unit without a sweeping loop");
     const LoopB &l0 = unit._block_list[pos].getLoop();
     std::vector<InstrPtr> sweeps;
     for (const InstrPtr &i : jitk::allLocalInstr(l0))

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <deque>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -165,6 +166,16 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     if (cache_dir.empty() || cache_dir == "NONE") cache_dir = default_cache_dir();
     if (cache_dir == "off") cache_dir.clear();
     pool.limit = cfg.defaultGet<uint64_t>("malloc_cache_limit_bytes", 64ull << 30);
+    twin_updates = cfg.defaultGet<bool>("twin_updates", true);
+    twin_min_bytes = cfg.defaultGet<uint64_t>("twin_min_bytes", 1u << 20);
+    tuner.enabled = cfg.defaultGet<bool>("autotune", true) && !dry_run;
+    tuner.samples = std::max(1, cfg.defaultGet<int>("autotune_samples", 3));
+    tuner.min_items = cfg.defaultGet<uint64_t>("autotune_min_items", 1u << 20);
+    tuner.min_gain = cfg.defaultGet<double>("autotune_min_gain", 0.02);
+    if (!cache_dir.empty() && cfg.defaultGet<bool>("autotune_persist", true)) {
+        tuner.path = cache_dir + "/autotune_" + cfg.defaultGet<std::string>("dry_run_arch", "gfx950") + ".txt";
+        tuner.load();
+    }
     if (dry_run) return;
     void *c = nullptr;
     check(bhhip_malloc(&c, 256), "allocation of counters");
@@ -215,6 +226,7 @@ void EngineHip::ensureResident(bh_base *base) {
         stats.h2d_bytes += b.bytes;
         stats.time_copy += seconds_since(t);
         b.dev_valid = true;
+        ++b.writes;
     }
 }
 
@@ -311,6 +323,31 @@ void EngineHip::release(bh_base *base) {
     auto it = bufs.find(base);
     if (it != bufs.end()) eraseBuffer(it);
     bh_data_free(base);
+    auto tw = twins.find(base);
+    if (tw != twins.end()) {
+        auto tb = bufs.find(tw->second.base.get());
+        if (tb != bufs.end()) eraseBuffer(tb);
+        twins.erase(tw);
+    }
+}
+
+void EngineHip::exchangeTwin(const TwinUpdate &u) {
+    DeviceBuffer &g = buffer(u.target);
+    DeviceBuffer &t = buffer(u.twin);
+    unguardHost(g);
+    std::swap(g.ptr, t.ptr);
+    noteWrite(u.target, g);
+    noteWrite(u.twin, t);
+    twins[u.target].in_step_at = u.in_step_after ? g.writes : ~0ull;
+    flush_exchanges.push_back(u);
+    ++twin_exchanges;
+}
+
+void EngineHip::applyEffects(const FlushEffects &f) {
+    for (bh_base *b : f.written) ++buffer(b).writes;
+    for (const TwinUpdate &u : f.exchanges) exchangeTwin(u);
+    flush_written.clear();
+    flush_exchanges.clear();
 }
 
 void EngineHip::ensureScratch(unsigned char *&ptr, uint64_t &have, uint64_t need, bool zero) {
@@ -342,6 +379,7 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
                               std::to_string(plan.block) + " " + hex);
         ++stats.launches;
         ++stats.launches_by_strategy[strategy_name(plan.strategy)];
+        for (int o : plan.written_ordinals) noteWrite(ord2base[o], buffer(ord2base[o]));
         static const char *dump = std::getenv("BH_HIP_DUMP_DIR");
         if (dump != nullptr && *dump) dumpLaunch(dump, plan, ord2base, instrs);
         return;
@@ -374,11 +412,7 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
         if (recording != nullptr) recording->push_back({plan.fin_function, 1, plan.block, {buf, buf + n}});
     }
     ++stats.launches_by_strategy[strategy_name(plan.strategy)];
-    for (int o : plan.written_ordinals) {
-        DeviceBuffer &d = buffer(ord2base[o]);
-        d.dev_valid = true;
-        d.host_valid = false;
-    }
+    for (int o : plan.written_ordinals) noteWrite(ord2base[o], buffer(ord2base[o]));
     if (profile) {
         bhhip_event_record(ev1);
         char hex[32];
@@ -630,11 +664,33 @@ void EngineHip::handleExecution(BhIR *bhir) {
     const auto t_total = std::chrono::steady_clock::now();
     ++stats.flushes;
     released_now.clear();
+    flush_written.clear();
+    flush_exchanges.clear();
     std::set<bh_base *> frees;
     std::vector<bh_instruction *> instr_list = jitk::remove_non_computed_system_instr(bhir->instr_list, frees);
     for (bh_base *b : frees) release(b);
     rewrite_store.clear();
     if (unflatten) unflatten_full_reductions(instr_list, rewrite_store);
+    std::vector<TwinUpdate> exchanges;
+    if (twin_updates) {
+        std::set<bh_base *> syncs;
+        for (bh_base *b : bhir->getSyncs()) syncs.insert(b);
+        auto twin_for = [&](bh_base *g) -> bh_base * {
+            if (static_cast<uint64_t>(g->nbytes()) < twin_min_bytes) return nullptr;
+            auto b = bufs.find(g);
+            if (b != bufs.end() && b->second.external) return nullptr;
+            Twin &tw = twins[g];
+            if (!tw.base) tw.base.reset(new bh_base(g->nelem(), g->dtype()));
+            return tw.base.get();
+        };
+        auto in_step = [&](bh_base *g) {
+            auto tw = twins.find(g);
+            auto b = bufs.find(g);
+            return tw != twins.end() && b != bufs.end() && tw->second.in_step_at == b->second.writes;
+        };
+        exchanges = rewrite_in_place_updates(instr_list, rewrite_store, syncs, twin_for, in_step);
+        for (const TwinUpdate &u : exchanges) twin_full_copies += u.full_copy ? 1 : 0;
+    }
     jitk::set_constructor_flag(instr_list);
 
     auto t = std::chrono::steady_clock::now();
@@ -646,6 +702,17 @@ void EngineHip::handleExecution(BhIR *bhir) {
     if (hit == flush_cache.end()) {
         ++flush_cache_misses;
         std::vector<jitk::LoopB> kernels = fold_serial_tails(jitk::get_kernel_list(instr_list, fusion_config));
+        // A kernel releases the bases freed in it when it ends: no later kernel may use them.
+        {
+            std::set<const bh_base *> released;
+            for (const jitk::LoopB &k : kernels) {
+                for (const bh_base *b : k.getAllBases())
+                    if (released.count(b))
+                        throw std::logic_error("hip engine: a kernel uses a base that an earlier kernel frees:\n" +
+                                               k.pprint());
+                for (bh_base *b : k.getAllFrees()) released.insert(b);
+            }
+        }
         stats.time_fuse += seconds_since(t);
         t = std::chrono::steady_clock::now();
         CachedFlush cf;
@@ -657,24 +724,29 @@ void EngineHip::handleExecution(BhIR *bhir) {
                 if (!bad.empty())
                     throw std::logic_error("hip engine: kernel loops do not match its views: " + bad + "\n" +
                                            kernel.pprint());
-                ck.plans = plan_kernel(kernel, wcfg);
                 const std::vector<float> feats = block_features(jitk::allInstr(kernel));
-                for (KernelPlan &p : ck.plans) p.features = feats;
-                for (KernelPlan &p : ck.plans) {
-                    for (const ArgSpec &a : p.args)
-                        if (a.view_origin >= static_cast<int64_t>(instr_list.size())) p.reusable = false;
-                    cf.reusable &= p.reusable;
-                    for (bh_base *b : p.params) p.param_ordinals.push_back(base2ord.at(b));
-                    for (bh_base *b : p.written) p.written_ordinals.push_back(base2ord.at(b));
-                    for (ArgSpec &a : p.args)
-                        if (a.kind == ArgSpec::BASE_PTR) a.base_ordinal = base2ord.at(a.base);
-                    p.source_hash = fnv1a(p.source);
-                    cf.index_checks |= p.uses_index_check;
-                    if (debug)
-                        std::fprintf(stderr, "[hip] plan %s grid %llu\n%s\n", strategy_name(p.strategy),
-                                     static_cast<unsigned long long>(p.grid),
-                                     std::getenv("BH_HIP_DEBUG")[0] == '2' ? p.source.c_str() : "");
-                }
+                // Ordinals, features and hashes of a kernel's plans (under any knob set).
+                auto finish = [&](std::vector<KernelPlan> &plans) {
+                    for (KernelPlan &p : plans) {
+                        p.features = feats;
+                        for (const ArgSpec &a : p.args)
+                            if (a.view_origin >= static_cast<int64_t>(instr_list.size())) p.reusable = false;
+                        cf.reusable &= p.reusable;
+                        for (bh_base *b : p.params) p.param_ordinals.push_back(base2ord.at(b));
+                        for (bh_base *b : p.written) p.written_ordinals.push_back(base2ord.at(b));
+                        for (ArgSpec &a : p.args)
+                            if (a.kind == ArgSpec::BASE_PTR) a.base_ordinal = base2ord.at(a.base);
+                        p.source_hash = fnv1a(p.source);
+                        cf.index_checks |= p.uses_index_check;
+                        if (debug)
+                            std::fprintf(stderr, "[hip] plan %s grid %llu\n%s\n", strategy_name(p.strategy),
+                                         static_cast<unsigned long long>(p.grid),
+                                         std::getenv("BH_HIP_DEBUG")[0] == '2' ? p.source.c_str() : "");
+                    }
+                };
+                ck.plans = plan_kernel(kernel, wcfg);
+                finish(ck.plans);
+                if (tuner.enabled) tuneKernel(ck, kernel, finish);
             }
             for (bh_base *b : kernel.getAllFrees()) ck.free_ordinals.push_back(base2ord.at(b));
             cf.kernels.push_back(std::move(ck));
@@ -697,9 +769,18 @@ void EngineHip::handleExecution(BhIR *bhir) {
         for (CachedKernel &ck : cf.kernels) {
             ++stats.kernels;
             ++kernel_seq;
-            for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
+            if (ck.trial && !settleTrial(ck)) {
+                runTrial(ck, ord2base, instr_list);
+            } else {
+                for (KernelPlan &p : ck.plans) launchPlan(p, ord2base, instr_list);
+            }
             for (int o : ck.free_ordinals) release(ord2base[o]);
         }
+    }
+    // The updated arrays take the twins' buffers (their new values); the twins keep the old ones.
+    for (const TwinUpdate &u : exchanges) {
+        exchangeTwin(u);
+        release(u.temp);
     }
     // The bridge deletes a freed base after this flush and may reuse its address for a new base;
     // a buffer still filed under that address would then be handed to the new base.
@@ -724,6 +805,142 @@ void EngineHip::handleExecution(BhIR *bhir) {
         }
     }
     stats.time_total += seconds_since(t_total);
+}
+
+// ------------------------------------------------------------------ autotuner
+
+EngineHip::Trial::~Trial() {
+    for (Pending &p : pending) {
+        bhhip_event_destroy(p.ev0);
+        bhhip_event_destroy(p.ev1);
+    }
+}
+
+void EngineHip::tuneKernel(CachedKernel &ck, const jitk::LoopB &kernel,
+                           const std::function<void(std::vector<KernelPlan> &)> &finish) {
+    const std::string key = tuner.signature(ck.plans);
+    if (key.empty()) return;
+    const LaunchKnobs configured = LaunchKnobs::of(wcfg);
+    if (const Tuner::Entry *e = tuner.lookup(key)) {
+        ++tuner.table_hits;
+        if (!(e->knobs == configured)) {
+            std::vector<KernelPlan> plans = plan_kernel(kernel, e->knobs.apply(wcfg));
+            finish(plans);
+            ck.plans = std::move(plans);
+        }
+        return;
+    }
+    const KernelPlan *big = &ck.plans.front();
+    for (const KernelPlan &p : ck.plans)
+        if (p.work_items > big->work_items) big = &p;
+    auto t = std::make_shared<Trial>();
+    t->key = key;
+    t->knobs.push_back(configured);
+    t->variants.push_back(ck.plans);
+    for (const LaunchKnobs &k : tuner.candidates(big->strategy, configured)) {
+        if (k == configured) continue;
+        std::vector<KernelPlan> plans = plan_kernel(kernel, k.apply(wcfg));
+        finish(plans);
+        bool dup = false;
+        for (const std::vector<KernelPlan> &v : t->variants) dup |= same_launches(v, plans);
+        if (dup) continue;
+        t->knobs.push_back(k);
+        t->variants.push_back(std::move(plans));
+    }
+    if (t->variants.size() < 2) return;
+    t->ms.resize(t->variants.size());
+    t->inflight.assign(t->variants.size(), 0);
+    ck.trial = std::move(t);
+}
+
+void EngineHip::pollTrial(Trial &t, bool wait) {
+    size_t keep = 0;
+    for (size_t i = 0; i < t.pending.size(); ++i) {
+        Trial::Pending &p = t.pending[i];
+        int done = 1;
+        if (!wait) check(bhhip_event_done(p.ev1, &done), "event query");
+        if (!done) {
+            t.pending[keep++] = p;
+            continue;
+        }
+        float ms = 0;
+        check(bhhip_event_elapsed_ms(p.ev0, p.ev1, &ms), "event timing");
+        t.ms[p.cand].push_back(ms);
+        --t.inflight[p.cand];
+        bhhip_event_destroy(p.ev0);
+        bhhip_event_destroy(p.ev1);
+    }
+    t.pending.resize(keep);
+}
+
+// Adopt the table's choice when another flush settled the signature meanwhile, else the fastest
+// candidate once every one has its samples.  True when the kernel's plans are final.
+bool EngineHip::settleTrial(CachedKernel &ck) {
+    Trial &t = *ck.trial;
+    pollTrial(t, false);
+    if (const Tuner::Entry *e = tuner.lookup(t.key)) {
+        size_t pick = 0;
+        for (size_t c = 0; c < t.knobs.size(); ++c)
+            if (t.knobs[c] == e->knobs) pick = c;
+        ck.plans = t.variants[pick];
+        ck.trial.reset();
+        return true;
+    }
+    for (size_t c = 0; c < t.ms.size(); ++c)
+        if (static_cast<int>(t.ms[c].size()) < tuner.samples) return false;
+    std::vector<double> best(t.ms.size());
+    for (size_t c = 0; c < t.ms.size(); ++c) best[c] = *std::min_element(t.ms[c].begin(), t.ms[c].end());
+    size_t pick = 0;
+    for (size_t c = 1; c < best.size(); ++c)
+        if (best[c] < best[pick]) pick = c;
+    if (pick != 0 && best[pick] > best[0] * (1.0 - tuner.min_gain)) pick = 0;
+    if (pick == 0) ++tuner.kept_default;
+    ++tuner.decisions;
+    tuner.saved_ms += best[0] - best[pick];
+    Tuner::Entry e;
+    e.knobs = t.knobs[pick];
+    e.best_ms = best[pick];
+    e.default_ms = best[0];
+    tuner.store(t.key, e);
+    ck.plans = t.variants[pick];
+    ck.trial.reset();
+    return true;
+}
+
+void EngineHip::runTrial(CachedKernel &ck, const std::vector<bh_base *> &ord2base,
+                         const std::vector<bh_instruction *> &instrs) {
+    Trial &t = *ck.trial;
+    size_t c = 0;
+    for (size_t i = 1; i < t.variants.size(); ++i)
+        if (t.ms[i].size() + t.inflight[i] < t.ms[c].size() + t.inflight[c]) c = i;
+    // Compile and make the operands resident before the first event: the events time the device's
+    // timeline, so host work between them (a hiprtc compile, an upload) would count as kernel time.
+    for (KernelPlan &plan : t.variants[c]) {
+        if (plan.function == nullptr) compilePlan(plan);
+        for (int o : plan.param_ordinals) ensureResident(ord2base[o]);
+        if (plan.strategy == Strategy::FULL_REDUCE)
+            ensureScratch(reduce_scratch, reduce_scratch_bytes, plan.scratch_bytes, false);
+        if (plan.strategy == Strategy::SCAN) ensureScratch(scan_scratch, scan_scratch_bytes, plan.scratch_bytes, true);
+    }
+    Trial::Pending p{c, nullptr, nullptr};
+    check(bhhip_event_create(&p.ev0), "event creation");
+    check(bhhip_event_create(&p.ev1), "event creation");
+    check(bhhip_event_record(p.ev0), "event record");
+    for (KernelPlan &plan : t.variants[c]) launchPlan(plan, ord2base, instrs);
+    check(bhhip_event_record(p.ev1), "event record");
+    t.pending.push_back(p);
+    ++t.inflight[c];
+    ++tuner.trials;
+}
+
+void EngineHip::resolveTuning() {
+    if (dry_run) return;
+    for (auto &kv : flush_cache)
+        for (CachedKernel &ck : kv.second.kernels)
+            if (ck.trial) {
+                pollTrial(*ck.trial, true);
+                settleTrial(ck);
+            }
 }
 
 void EngineHip::handleExtmethod(BhIR *bhir) {
@@ -751,9 +968,7 @@ void EngineHip::handleExtmethod(BhIR *bhir) {
             // the device copies of all of them are now the current ones.
             for (const bh_view &v : instr.operand) {
                 if (v.isConstant()) continue;
-                DeviceBuffer &d = buffer(v.base);
-                d.dev_valid = true;
-                d.host_valid = false;
+                noteWrite(v.base, buffer(v.base));
             }
         } else if (comp.child_extmethods.count(instr.opcode)) {
             throw std::runtime_error("hip engine: extension method is only known to a child component");
@@ -885,6 +1100,7 @@ void EngineHip::markDeviceWritten(bh_base *base) {
     DeviceBuffer &d = buffer(base);
     d.dev_valid = true;
     d.host_valid = false;
+    ++d.writes;
 }
 
 void EngineHip::setMemoryPointer(bh_base *base, bool host_ptr, void *mem) {
@@ -902,6 +1118,7 @@ void EngineHip::setMemoryPointer(bh_base *base, bool host_ptr, void *mem) {
         d.external = true;
         d.dev_valid = true;
         d.host_valid = false;
+        ++d.writes;
     }
 }
 
@@ -934,6 +1151,10 @@ std::string EngineHip::statistics() const {
     ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
     ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";
+    ss << "twin_exchanges: " << twin_exchanges << "\ntwin_full_copies: " << twin_full_copies << "\n";
+    ss << "autotune_trials: " << tuner.trials << "\nautotune_decisions: " << tuner.decisions
+       << "\nautotune_kept_configured: " << tuner.kept_default << "\nautotune_table_hits: " << tuner.table_hits
+       << "\nautotune_saved_ms: " << tuner.saved_ms << "\n";
     ss << "pool_allocated: " << pool.allocated << "\npool_peak: " << pool.peak << "\npool_hits: " << pool.hits
        << "\npool_misses: " << pool.misses << "\n";
     return ss.str();
@@ -1010,16 +1231,30 @@ public:
         }
         bh_base *cond = bhir->getRepeatCondition();
         const uint64_t nrepeats = bhir->getNRepeats();
-        // A repeat body that only launches kernels, with the same arguments in two successive
-        // iterations, is captured once as a graph and replayed for the remaining iterations.
+        // A repeat body that only launches kernels is captured as a graph and replayed for the
+        // remaining iterations: once two successive iterations launched the same sequence (period
+        // 1), or once an iteration repeats the one before the previous (period 2: an array updated
+        // in place through its twin alternates between two buffers).  A replayed iteration also
+        // repeats what its flush did to engine state besides launching (write counts, twin
+        // exchanges).
         const bool graphable = engine.graph_repeats && !engine.profile && !engine.dry_run && cond == nullptr &&
                                nrepeats >= 4 && !has_slides(*bhir) && !has_extmethods(*bhir);
-        std::vector<EngineHip::IssuedLaunch> prev, cur;
-        void *graph = nullptr;
-        size_t graph_launches = 0;
+        struct Iteration {
+            std::vector<EngineHip::IssuedLaunch> launches;
+            bool quiet;
+            EngineHip::FlushEffects effects;
+        };
+        std::deque<Iteration> hist;
+        std::vector<EngineHip::IssuedLaunch> cur;
+        void *graphs[2] = {nullptr, nullptr};
+        size_t graph_launches[2] = {0, 0};
+        EngineHip::FlushEffects graph_effects[2];
+        int period = 0, phase = 0;
         for (uint64_t i = 0; i < nrepeats; ++i) {
-            if (graph != nullptr) {
-                engine.graphLaunch(graph, graph_launches);
+            if (period > 0) {
+                engine.graphLaunch(graphs[phase], graph_launches[phase]);
+                engine.applyEffects(graph_effects[phase]);
+                phase = (phase + 1) % period;
                 continue;
             }
             const bool rec = graphable && i >= 1;
@@ -1034,18 +1269,39 @@ public:
                 throw;
             }
             engine.recording = nullptr;
+            EngineHip::FlushEffects fx = engine.takeEffects();
             if (rec) {
-                if (i >= 2 && !cur.empty()) {
-                    if (cur != prev) {
-                        ++engine.graph_declined_changed;
-                    } else if (engine.quietToken() != q0) {
-                        ++engine.graph_declined_busy;
+                hist.push_back({cur, engine.quietToken() == q0, std::move(fx)});
+                if (hist.size() > 3) hist.pop_front();
+                const size_t n = hist.size();
+                const Iteration &last = hist[n - 1];
+                if (n >= 2 && !last.launches.empty()) {
+                    if (last.launches == hist[n - 2].launches) {
+                        if (!last.quiet) {
+                            ++engine.graph_declined_busy;
+                        } else {
+                            graphs[0] = engine.graphFor(last.launches);
+                            graph_launches[0] = last.launches.size();
+                            graph_effects[0] = last.effects;
+                            period = 1;
+                        }
+                    } else if (n >= 3 && last.launches == hist[n - 3].launches) {
+                        if (!last.quiet || !hist[n - 2].quiet) {
+                            ++engine.graph_declined_busy;
+                        } else {
+                            // the next iteration repeats hist[n - 2], the one after it `last`
+                            graphs[0] = engine.graphFor(hist[n - 2].launches);
+                            graph_launches[0] = hist[n - 2].launches.size();
+                            graph_effects[0] = hist[n - 2].effects;
+                            graphs[1] = engine.graphFor(last.launches);
+                            graph_launches[1] = last.launches.size();
+                            graph_effects[1] = last.effects;
+                            period = 2;
+                        }
                     } else {
-                        graph = engine.graphFor(cur);
-                        graph_launches = cur.size();
+                        ++engine.graph_declined_changed;
                     }
                 }
-                prev.swap(cur);
             }
             if (cond != nullptr) {
                 engine.copyToHost(cond);
@@ -1136,6 +1392,23 @@ public:
             engine.setBatchMin(std::stoi(msg.substr(10)));
         } else if (msg == "plans: drop") {
             engine.dropPlans();
+        } else if (msg == "autotune: on" || msg == "autotune: off") {
+            // Tuning applies to kernels planned from here on; cached plans are dropped.
+            engine.tuner.enabled = msg == "autotune: on" && !engine.dry_run;
+            engine.dropPlans();
+        } else if (msg == "autotune: forget") {
+            // Forget the in-memory table (its file is kept) and every cached plan.
+            engine.tuner.clear();
+            engine.dropPlans();
+        } else if (msg.rfind("autotune_samples:", 0) == 0) {
+            engine.tuner.samples = std::max(1, std::stoi(msg.substr(17)));
+        } else if (msg.rfind("autotune_min_items:", 0) == 0) {
+            engine.tuner.min_items = std::stoull(msg.substr(19));
+            engine.dropPlans();
+        } else if (msg == "autotune") {
+            // Settle what can be settled (synchronizes), then list the table.
+            engine.resolveTuning();
+            ss << engine.tuner.dump();
         } else if (msg == "host_write_tracking: on") {
             engine.track_host_writes = true;
         } else if (msg == "host_write_tracking: off") {

@@ -11,6 +11,7 @@
  */
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <set>
@@ -22,6 +23,7 @@
 #include "codegen.hpp"
 #include "rewrite.hpp"
 #include "features.hpp"
+#include "tuner.hpp"
 
 namespace bohrium {
 namespace vehip {
@@ -36,6 +38,8 @@ struct DeviceBuffer {
      * current; the first host write faults, and the fault handler makes the host copy the newest. */
     void *guard_addr = nullptr;
     uint64_t guard_bytes = 0;
+    /* Counts every change of the device copy's contents (launch, upload, exchange with a twin). */
+    uint64_t writes = 0;
 };
 
 /* Size-keyed reuse of device allocations (hipMalloc is far slower than a lookup). */
@@ -143,6 +147,30 @@ public:
         return wcfg;
     }
 
+    /* What a flush did to engine state besides its launches; a replayed graph of that flush
+     * repeats it (bases written: their write counts; twin exchanges). */
+    struct FlushEffects {
+        std::vector<bh_base *> written;
+        std::vector<TwinUpdate> exchanges;
+    };
+    FlushEffects takeEffects() {
+        FlushEffects f;
+        f.written.assign(flush_written.begin(), flush_written.end());
+        f.exchanges.swap(flush_exchanges);
+        flush_written.clear();
+        return f;
+    }
+    void applyEffects(const FlushEffects &f);
+    /* In-place updates through a temporary run as buffer exchanges with a twin (rewrite.hpp). */
+    bool twin_updates = true;
+    uint64_t twin_min_bytes = 1u << 20;
+    uint64_t twin_exchanges = 0, twin_full_copies = 0;
+
+    /* Launch shapes per kernel signature (see tuner.hpp). */
+    Tuner tuner;
+    /* Read every outstanding tuning measurement (synchronizes) and settle finished trials. */
+    void resolveTuning();
+
     jitk::FusionConfig fusion_config;
     bool profile = false;
     /* Compile-only mode: plan and hiprtc-compile every kernel, touch no device. */
@@ -158,6 +186,22 @@ private:
     bool resident = true;
     int num_cus = 0;  // compute units of the device (0 in compile-only mode)
     std::unordered_map<const bh_base *, DeviceBuffer> bufs;
+    /* The second buffer of an array updated in place (rewrite_in_place_updates), and the write
+     * count of the array at which the twin last equalled it outside the updated view. */
+    struct Twin {
+        std::unique_ptr<bh_base> base;
+        uint64_t in_step_at = ~0ull;
+    };
+    std::unordered_map<const bh_base *, Twin> twins;
+    std::set<bh_base *> flush_written;
+    std::vector<TwinUpdate> flush_exchanges;
+    void exchangeTwin(const TwinUpdate &u);
+    void noteWrite(bh_base *base, DeviceBuffer &d) {
+        d.dev_valid = true;
+        d.host_valid = false;
+        ++d.writes;
+        flush_written.insert(base);
+    }
     /* Bases released during the current flush; a BH_FREE that no kernel released is caught with it. */
     std::set<const bh_base *> released_now;
     DevicePool pool;
@@ -167,10 +211,26 @@ private:
     uint64_t scan_scratch_bytes = 0;
     unsigned int *counters = nullptr;  // [0]: reduce ticket, [1]: error flag
     EngineStats stats;
+    /* Autotuning of one kernel in progress: its plans under each distinct candidate knob set,
+     * device times measured so far, and launches whose events are not read yet. */
+    struct Trial {
+        std::string key;
+        std::vector<LaunchKnobs> knobs;  // [0]: the configured knobs
+        std::vector<std::vector<KernelPlan> > variants;
+        std::vector<std::vector<double> > ms;
+        std::vector<int> inflight;
+        struct Pending {
+            size_t cand;
+            void *ev0, *ev1;
+        };
+        std::vector<Pending> pending;
+        ~Trial();
+    };
     /* Plans of an already seen flush, keyed by its canonical encoding. */
     struct CachedKernel {
         std::vector<KernelPlan> plans;
         std::vector<int> free_ordinals;
+        std::shared_ptr<Trial> trial;  // set while the kernel's launch shape is being tuned
     };
     /* Many launches of one kernel code in a flush run as one batched launch (bh_kernel_batch):
      * each item keeps its own packed arguments and workgroup count.  Launches are ordered in
@@ -231,6 +291,14 @@ private:
                          std::unordered_map<const bh_base *, int> &base2ord) const;
     void launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2base,
                     const std::vector<bh_instruction *> &instrs);
+    /* Autotuner: set up a trial for a kernel just planned (or take the table's choice), run one
+     * launch of the next candidate, read finished measurements, and settle on a candidate. */
+    void tuneKernel(CachedKernel &ck, const jitk::LoopB &kernel,
+                    const std::function<void(std::vector<KernelPlan> &)> &finish);
+    void runTrial(CachedKernel &ck, const std::vector<bh_base *> &ord2base,
+                  const std::vector<bh_instruction *> &instrs);
+    void pollTrial(Trial &t, bool wait);
+    bool settleTrial(CachedKernel &ck);
     struct PendingEvent {
         void *start, *stop;
         LaunchRecord rec;

@@ -91,3 +91,126 @@ int unflatten_full_reductions(std::vector<bh_instruction *> &list, RewriteStore 
 
 } // namespace vehip
 } // namespace bohrium
+
+namespace bohrium {
+namespace vehip {
+
+namespace {
+
+bool touches(const bh_instruction *i, const bh_base *b) {
+    for (const bh_view &v : i->operand)
+        if (!v.isConstant() && v.base == b) return true;
+    return false;
+}
+
+// Every computing instruction of the flushes considered writes only its operand 0 (flushes with
+// extension methods are left alone).
+bool writes(const bh_instruction *i, const bh_base *b) {
+    return i->opcode != BH_FREE && !i->operand.empty() && !i->operand[0].isConstant() && i->operand[0].base == b;
+}
+
+bool same_shape(const bh_view &a, const bh_view &b) {
+    if (a.ndim != b.ndim) return false;
+    for (int64_t d = 0; d < a.ndim; ++d)
+        if (a.shape[d] != b.shape[d]) return false;
+    return true;
+}
+
+constexpr size_t kNone = static_cast<size_t>(-1);
+
+} // namespace
+
+std::vector<TwinUpdate> rewrite_in_place_updates(std::vector<bh_instruction *> &list, RewriteStore &store,
+                                                 const std::set<bh_base *> &syncs,
+                                                 const std::function<bh_base *(bh_base *)> &twin_for,
+                                                 const std::function<bool(bh_base *)> &in_step) {
+    std::vector<TwinUpdate> out;
+    for (const bh_instruction *i : list) {
+        if (i->opcode > BH_MAX_OPCODE_ID) return out;
+        for (const bh_view &v : i->operand)
+            if (!v.isConstant() && v.hasSlide()) return out;
+    }
+    std::set<const bh_base *> done;
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (size_t q = 0; q < list.size() && !changed; ++q) {
+            const bh_instruction *cp = list[q];
+            if (cp->opcode != BH_IDENTITY || cp->operand.size() != 2) continue;
+            const bh_view &V = cp->operand[0], &in = cp->operand[1];
+            if (V.isConstant() || in.isConstant()) continue;
+            bh_base *G = V.base, *T = in.base;
+            if (G == T || done.count(G) || done.count(T) || G->dtype() != T->dtype() || syncs.count(T)) continue;
+            if (!full_contiguous(in) || !same_shape(in, V)) continue;
+            size_t free_t = kNone;
+            bool free_g = false;
+            for (size_t k = 0; k < list.size(); ++k) {
+                if (list[k]->opcode != BH_FREE) continue;
+                if (list[k]->operand[0].base == T && k > q) free_t = k;
+                if (list[k]->operand[0].base == G) free_g = true;
+            }
+            if (free_t == kNone || free_g) continue;
+            size_t p0 = kNone;
+            for (size_t k = 0; k < q && p0 == kNone; ++k)
+                if (touches(list[k], T)) p0 = k;
+            if (p0 == kNone || !writes(list[p0], T)) continue;
+            bool ok = true;
+            for (size_t o = 1; o < list[p0]->operand.size(); ++o) {
+                const bh_view &v = list[p0]->operand[o];
+                if (!v.isConstant() && v.base == T) ok = false;  // T's first write does not read T
+            }
+            for (size_t k = p0; k <= q && ok; ++k)
+                for (const bh_view &v : list[k]->operand)
+                    if (!v.isConstant() && v.base == T && !(full_contiguous(v) && same_shape(v, V))) ok = false;
+            for (size_t k = q + 1; k < list.size() && ok; ++k)
+                if (k != free_t && touches(list[k], T)) ok = false;
+            for (size_t k = p0; k < q && ok; ++k)
+                if (writes(list[k], G)) ok = false;
+            if (!ok) continue;
+            bh_base *twin = twin_for(G);
+            if (twin == nullptr) continue;
+            TwinUpdate u;
+            u.target = G;
+            u.twin = twin;
+            u.temp = T;
+            bool written_before = false;
+            for (size_t k = 0; k < p0; ++k) written_before |= writes(list[k], G);
+            u.full_copy = written_before || !in_step(G);
+            u.in_step_after = true;
+            for (size_t k = q + 1; k < list.size(); ++k) u.in_step_after &= !writes(list[k], G);
+
+            const bh_view at_c(twin, V.start, V.ndim, V.shape, V.stride);
+            std::vector<bh_instruction *> nl;
+            nl.reserve(list.size() + 1);
+            for (size_t k = 0; k < list.size(); ++k) {
+                if (k == p0 && u.full_copy) {
+                    store.instrs.push_back(bh_instruction(BH_IDENTITY, {bh_view(twin), bh_view(G)}));
+                    nl.push_back(&store.instrs.back());
+                }
+                if (k == q || k == free_t) continue;
+                bh_instruction *i = list[k];
+                const bool rename = (k >= p0 && k < q && touches(i, T)) || (k > q && touches(i, G));
+                if (!rename) {
+                    nl.push_back(i);
+                    continue;
+                }
+                store.instrs.push_back(*i);
+                bh_instruction *c = &store.instrs.back();
+                for (bh_view &v : c->operand) {
+                    if (v.isConstant()) continue;
+                    if (k < q && v.base == T) v = at_c;
+                    else if (k > q && v.base == G) v.base = twin;
+                }
+                nl.push_back(c);
+            }
+            list.swap(nl);
+            done.insert(G);
+            done.insert(twin);
+            out.push_back(u);
+            changed = true;
+        }
+    }
+    return out;
+}
+
+} // namespace vehip
+} // namespace bohrium

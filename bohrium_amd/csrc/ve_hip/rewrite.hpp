@@ -14,6 +14,8 @@
 #pragma once
 
 #include <deque>
+#include <functional>
+#include <set>
 #include <memory>
 #include <vector>
 
@@ -34,6 +36,29 @@ struct RewriteStore {
 
 /* Returns the number of reductions rewritten. */
 int unflatten_full_reductions(std::vector<bh_instruction *> &list, RewriteStore &store);
+
+/* In-place updates through a temporary, as the front ends emit them for `c[...] = f(views of G)`
+ * with c a view of G: T = f(...) (T a temporary of c's shape), IDENTITY(c, T), BH_FREE(T).  Run as
+ * written, T makes a round trip through device memory and the copy reads and writes c again.
+ * The rewrite computes T's elements straight into a second buffer of G's size (the twin, at c's
+ * places), removes the copy, and lets every later instruction of the flush use the twin for G;
+ * after the flush the engine exchanges the two buffers, so G holds the new values and the twin the
+ * old ones.  The twin must equal G outside c; when the engine cannot vouch for that (`in_step`
+ * false, or G is written earlier in the flush) a full copy G -> twin is inserted first.
+ * Conditions: T is written first by a full write of its whole shape and every access of T up to
+ * the copy is that full view, T is not synced and is freed in the flush, the copy converts no type,
+ * G is not written between T's first write and the copy, no view slides, G is not freed. */
+struct TwinUpdate {
+    bh_base *target = nullptr;  // G
+    bh_base *twin = nullptr;
+    bh_base *temp = nullptr;    // T: no longer used by the flush; its BH_FREE is removed, the engine releases it
+    bool full_copy = false;           // a copy G -> twin was inserted
+    bool in_step_after = false;       // after the exchange, twin == G outside c (nothing else wrote G)
+};
+std::vector<TwinUpdate> rewrite_in_place_updates(std::vector<bh_instruction *> &list, RewriteStore &store,
+                                                 const std::set<bh_base *> &syncs,
+                                                 const std::function<bh_base *(bh_base *)> &twin_for,
+                                                 const std::function<bool(bh_base *)> &in_step);
 
 } // namespace vehip
 } // namespace bohrium

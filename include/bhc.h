@@ -83,6 +83,17 @@ int bhc_extmethod(const char *name, int nops, const bhc_view_t *views);
  * front ends do (bridge/bh107/bh107/user_kernel.py:88-90). */
 const char *bhc_user_kernel(const char *kernel, int nops, const bhc_view_t *operands, const char *compile_cmd,
                             const char *tag, const char *param);
+/* Recorded flushes (no reference C-ABI counterpart; the format restates what the reference's
+ * BhIR serializer writes, core/bh_ir.cpp:37-154, without Boost; see core/bh_ir.cpp here).
+ * bhc_record_flushes: flush what is queued, then append every later flush to `path` (a new file)
+ * as a BhIR archive plus the data of the bases it introduces that hold host data; NULL or "" stops.
+ * bhc_replay_file: execute the flushes recorded in `path` through the stack, in order (what this
+ * process queued is flushed first).  bhc_replay_base: the base that stands in for the one the
+ * recording process had at `recorded_address` (NULL when none, or when it was freed); it belongs
+ * to the replay: read it (bhc_sync + bhc_flush, bhc_data_copy_out), never bhc_free it. */
+int bhc_record_flushes(const char *path);
+int bhc_replay_file(const char *path);
+bhc_base_t bhc_replay_base(uint64_t recorded_address);
 /* Auto-flush after this many queued instructions (default 1000 like the reference; <= 0 disables). */
 int bhc_set_flush_threshold(int64_t n);
 int64_t bhc_base_nelem(bhc_base_t base);

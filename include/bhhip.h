@@ -83,6 +83,8 @@ void *bhhip_stream(void);
 int bhhip_event_create(void **event);
 int bhhip_event_record(void *event);
 int bhhip_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Without waiting: *done = 1 when the work recorded before `event` has finished, else 0. */
+int bhhip_event_done(void *event, int *done);
 /* Wait until the work recorded before `event` has finished. */
 int bhhip_event_synchronize(void *event);
 int bhhip_event_destroy(void *event);

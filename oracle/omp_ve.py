@@ -264,6 +264,8 @@ class _Gen:
         return out
 
     def arr(self, bid):
+        if getattr(self, "referenced", None) is not None:
+            self.referenced.add(bid)
         return "((%s*)d[%d])" % (CTYPE[self.dtypes[bid]], bid)
 
     def ref(self, view, idx_names):
@@ -478,35 +480,59 @@ def replay(trace, keep=None):
 
 
 # ---------------------------------------------------------------------------------------------
-# Fused OpenMP form (CPU baseline).  Runs of 1-D contiguous instructions of equal length are
-# emitted as one `#pragma omp parallel for` loop in which temporaries (created and freed inside
-# the run) are scalars and 1-D full reductions use OpenMP reduction clauses, like the kernels the
-# reference OpenMP engine generates for such blocks (ve/openmp/engine_openmp.cpp:274-329,
-# ve/openmp/openmp_util.hpp:27-94).  Everything else runs per instruction as in `replay`.
+# Fused OpenMP form (CPU baseline).  Runs of consecutive instructions over one iteration shape (any
+# rank, any strides) become one loop nest, written the way the reference OpenMP engine writes a
+# fused block (ve/openmp/engine_openmp.cpp:274-329 loopHeadWriter / writeHeader, openmp_util.hpp):
+#   * `#pragma omp parallel for` on the outermost loop.  A sweep of that loop goes into a
+#     `reduction(op:acc)` clause when its operator has one and its output is a single element
+#     (scalar-replaced); otherwise each update is guarded by `#pragma omp atomic` (add, multiply,
+#     bitwise) or `#pragma omp critical` (everything else), as openmp_reduce_compatible /
+#     openmp_atomic_compatible decide.
+#   * `#pragma omp simd` on the innermost loop when every sweep of that loop has a reduction
+#     operator and nothing in the block is atomic or critical (simd_compatible); sweeps of the
+#     innermost loop are scalar accumulators named in its simd reduction clause (for rank > 0).
+#     A one-dimensional block is both: `parallel for simd`.
+#   * Temporaries created and freed inside the run are scalars (array contraction); reductions over
+#     the first or the last axis of the block.
+# Everything else (gather, scatter, accumulate, reductions over middle axes, sliding views) runs
+# per instruction as in `replay`.
 
 _OMP_RED = {OP["ADD_REDUCE"]: "+", OP["MULTIPLY_REDUCE"]: "*", OP["MINIMUM_REDUCE"]: "min",
             OP["MAXIMUM_REDUCE"]: "max", OP["BITWISE_AND_REDUCE"]: "&", OP["BITWISE_OR_REDUCE"]: "|",
-            OP["BITWISE_XOR_REDUCE"]: "^", OP["LOGICAL_AND_REDUCE"]: "&&", OP["LOGICAL_OR_REDUCE"]: "||"}
-_NO_FUSE = {OP["GATHER"], OP["SCATTER"], OP["COND_SCATTER"], OP["ADD_ACCUMULATE"], OP["MULTIPLY_ACCUMULATE"]}
+            OP["BITWISE_XOR_REDUCE"]: "^"}
+_OMP_ATOMIC = {OP["ADD_REDUCE"]: "+=", OP["MULTIPLY_REDUCE"]: "*=", OP["BITWISE_AND_REDUCE"]: "&=",
+               OP["BITWISE_OR_REDUCE"]: "|=", OP["BITWISE_XOR_REDUCE"]: "^="}
+_NO_FUSE = {OP["GATHER"], OP["SCATTER"], OP["COND_SCATTER"], OP["ADD_ACCUMULATE"], OP["MULTIPLY_ACCUMULATE"],
+            OP["FREE"], OP["NONE"], OP["TALLY"]}
 
 
-def _is_1d_unit(view, n):
-    return view is not None and len(view[2]) == 1 and view[2][0] == n and view[3][0] == 1
+def _iter_shape(rec):
+    ops = rec[2]
+    v = ops[1] if rec[1] in REDUCE else ops[0]
+    return tuple(v[2])
 
 
-def _fusible(rec, n):
+def _fusible(rec):
     opcode, ops, const = rec[1], rec[2], rec[3]
-    if opcode in _NO_FUSE:
+    if opcode in _NO_FUSE or opcode not in NAME:
+        return False
+    if len(rec) > 4 and rec[4] and any(rec[4]):
+        return False
+    S = _iter_shape(rec)
+    if len(S) == 0 or 0 in S:
         return False
     if opcode in REDUCE:
-        return opcode in _OMP_RED and len(ops[1][2]) == 1 and _is_1d_unit(ops[1], n)
-    return all(o is None or _is_1d_unit(o, n) for o in ops)
+        axis = int(const[1])
+        if axis not in (0, len(S) - 1):
+            return False
+        out_n = int(np.prod([s for k, s in enumerate(S) if k != axis])) if len(S) > 1 else 1
+        return int(np.prod(ops[0][2])) == out_n and (len(S) == 1 or tuple(ops[0][2]) == tuple(
+            s for k, s in enumerate(S) if k != axis))
+    return all(o is None or tuple(o[2]) == S for o in ops)
 
 
-def _principal_len(rec):
-    opcode, ops = rec[1], rec[2]
-    v = ops[1] if opcode in REDUCE else ops[0]
-    return v[2][0] if len(v[2]) == 1 else None
+def _view_key(o):
+    return (o[0], o[1], tuple(o[2]), tuple(o[3]))
 
 
 class FusedProgram:
@@ -522,10 +548,15 @@ class FusedProgram:
         gen = _Gen(self.dtypes)
         gen.nelem = self.nelem
         src = [HEADER]
+        self.runs = []  # (number of instructions, iteration shape) of every fused run, for tests
+        self.arrays = []  # per group: the bases its code addresses as arrays (temporaries are scalars)
         for gi, (recs, _, _) in enumerate(self.groups):
+            gen.referenced = set()
             src.append("int g%d(void **d) {\n  int err = 0;" % gi)
             src.extend(self._group_source(gen, recs))
             src.append("  return err;\n}\n")
+            self.arrays.append(gen.referenced)
+        gen.referenced = None
         self.source = "\n".join(src)
         self.lib = _compile(self.source)
         self.data = {}
@@ -534,7 +565,6 @@ class FusedProgram:
     def _group_source(self, gen, recs):
         ops_recs = [r for r in recs if r[0] == "op"]
         frees = set(r[1] for r in recs if r[0] == "free")
-        # bases used per instruction position, to find run-local temporaries
         uses = {}
         for k, r in enumerate(ops_recs):
             for o in r[2]:
@@ -543,67 +573,120 @@ class FusedProgram:
         out, k = [], 0
         while k < len(ops_recs):
             r = ops_recs[k]
-            n = _principal_len(r)
-            if n is None or not _fusible(r, n):
+            if len(r) > 4 and r[4] and any(r[4]):
+                raise NotImplementedError("oracle fused form: sliding views (use replay)")
+            if not _fusible(r):
                 out.extend(gen.instr(r[1], r[2], r[3]))
                 k += 1
                 continue
-            run = [r]
-            red_outs = set([r[2][0][0]]) if r[1] in REDUCE else set()
-            j = k + 1
+            S = _iter_shape(r)
+            written, read, red_outs, read_first = {}, {}, set(), set()
+            run, j = [], k
             while j < len(ops_recs):
                 q = ops_recs[j]
-                if _principal_len(q) != n or not _fusible(q, n):
+                if not _fusible(q) or _iter_shape(q) != S:
                     break
-                if any(o is not None and o[0] in red_outs for o in q[2]):
+                ins = [o for o in q[2][1:] if o is not None]
+                outv = q[2][0]
+                clash = any(o[0] in red_outs or (o[0] in written and written[o[0]] != _view_key(o)) for o in ins)
+                clash |= outv[0] in red_outs
+                clash |= outv[0] in written and written[outv[0]] != _view_key(outv)
+                clash |= any(v != _view_key(outv) for v in read.get(outv[0], ()))
+                if clash:
                     break
+                for o in ins:
+                    read.setdefault(o[0], set()).add(_view_key(o))
+                    if o[0] not in written:
+                        read_first.add(o[0])
                 if q[1] in REDUCE:
-                    red_outs.add(q[2][0][0])
+                    red_outs.add(outv[0])
+                else:
+                    written[outv[0]] = _view_key(outv)
                 run.append(q)
                 j += 1
             idx = set(range(k, j))
-            out.extend(self._run_source(gen, run, n, frees, uses, idx))
+            temps = set(b for b in written
+                        if b in frees and uses[b] <= idx and b not in red_outs and b not in read_first)
+            self.runs.append((len(run), S))
+            out.extend(self._run_source(gen, run, S, temps))
             k = j
         return out
 
-    def _run_source(self, gen, run, n, frees, uses, idx):
-        temps = set()
-        for r in run:
-            if r[1] in REDUCE:
-                continue
-            b = r[2][0][0]
-            if b in frees and uses[b] <= idx:
-                temps.add(b)
-        accs = {}
-        for r in run:
-            if r[1] in REDUCE:
-                accs[r[2][0][0]] = r
-        L = ["  {"]
-        for b, r in accs.items():
-            t = np.dtype(self.dtypes[b])
-            L.append("    %s acc%d = %s;" % (CTYPE[t.str], b, c_literal(t, sweep_identity(r[1], t))))
-        clause = "".join(" reduction(%s:acc%d)" % (_OMP_RED[r[1]], b) for b, r in accs.items())
-        L.append("    #pragma omp parallel for%s" % clause)
-        L.append("    for (int64_t i = 0; i < %d; ++i) {" % n)
-        for b in sorted(temps):
-            L.append("      %s t%d;" % (CTYPE[self.dtypes[b]], b))
+    def _run_source(self, gen, run, S, temps):
+        n = len(S)
+        names = ["i%d" % a for a in range(n)]
+        inner = names[-1]
 
         def ref(o):
             if o[0] in temps:
                 return "t%d" % o[0]
-            return "%s[%d + i]" % (gen.arr(o[0]), o[1])
+            return "%s[%s]" % (gen.arr(o[0]), _index(o, names))
 
+        # sweeps: axis 0 (outermost loop) or the last axis (innermost loop)
+        outer_sweeps, inner_sweeps = [], []
+        for r in run:
+            if r[1] in REDUCE:
+                axis = int(r[3][1])
+                (outer_sweeps if axis == 0 else inner_sweeps).append(r)
+        clause, atomic, critical = [], set(), set()
+        for r in outer_sweeps:
+            scalar = n == 1
+            if r[1] in _OMP_RED and scalar:
+                clause.append(r)
+        clause_ids = set(id(r) for r in clause)
+        for r in outer_sweeps:
+            if id(r) in clause_ids:
+                continue
+            if r[1] in _OMP_ATOMIC:
+                atomic.add(id(r))
+            else:
+                critical.add(id(r))
+        simd = all(r[1] in _OMP_RED for r in inner_sweeps) and not atomic and not critical
+        if n == 1:
+            simd = simd and all(r[1] in _OMP_RED for r in outer_sweeps)
+
+        def out_ref(r, idx_names):
+            o = r[2][0]
+            return "%s[%s]" % (gen.arr(o[0]), _index(o, idx_names))
+
+        L = ["  {"]
+        # a sweep of the outermost loop starts from its identity (the reference's identity block)
+        for r in outer_sweeps:
+            t = np.dtype(self.dtypes[r[2][0][0]])
+            ident = c_literal(t, sweep_identity(r[1], t))
+            if id(r) in clause_ids:
+                L.append("    %s acc%d = %s;" % (CTYPE[t.str], r[2][0][0], ident))
+            else:
+                L.extend(gen.loops(list(S[1:]), ["%s = %s;" % (out_ref(r, names[1:]), ident)], names[1:]))
+        pragma = "parallel for" + (" simd" if n == 1 and simd else "")
+        pragma += "".join(" reduction(%s:acc%d)" % (_OMP_RED[r[1]], r[2][0][0]) for r in clause)
+        body = []
+        for b in sorted(temps):
+            body.append("%s t%d;" % (CTYPE[self.dtypes[b]], b))
         for r in run:
             opcode, ops, const = r[1], r[2], r[3]
             if opcode in REDUCE:
                 t = np.dtype(self.dtypes[ops[0][0]])
-                L.append("      " + c_operation(opcode, [t, t], ["acc%d" % ops[0][0], ref(ops[1])]))
+                axis = int(const[1])
+                if axis == n - 1 and n > 1:
+                    body.append(c_operation(opcode, [t, t], ["r%d" % ops[0][0], ref(ops[1])]))
+                elif id(r) in clause_ids:
+                    body.append(c_operation(opcode, [t, t], ["acc%d" % ops[0][0], ref(ops[1])]))
+                elif id(r) in atomic:
+                    body.append("{ %s v_ = %s;" % (CTYPE[t.str], ref(ops[1])))
+                    body.append("#pragma omp atomic")
+                    body.append("  %s %s v_; }" % (out_ref(r, names[1:] if n > 1 else []), _OMP_ATOMIC[opcode]))
+                else:
+                    body.append("{ %s v_ = %s;" % (CTYPE[t.str], ref(ops[1])))
+                    body.append("#pragma omp critical")
+                    body.append("  { " + c_operation(opcode, [t, t], [out_ref(r, names[1:] if n > 1 else []), "v_"])
+                                + " } }")
             elif opcode == OP["RANDOM"]:
-                s, key = const[1]
-                L.append("      %s = oracle_random123(%dULL, %dULL, (uint64_t)(%d + i));" % (ref(ops[0]), s, key,
-                                                                                          ops[0][1]))
+                s0, key = const[1]
+                body.append("%s = oracle_random123(%dULL, %dULL, (uint64_t)%s);" % (ref(ops[0]), s0, key,
+                                                                                   _index(ops[0], names)))
             elif opcode == OP["RANGE"]:
-                L.append("      %s = %d + i;" % (ref(ops[0]), ops[0][1]))
+                body.append("%s = %s;" % (ref(ops[0]), _index(ops[0], names)))
             else:
                 types, cops = [], []
                 for o in ops:
@@ -613,10 +696,33 @@ class FusedProgram:
                     else:
                         types.append(np.dtype(self.dtypes[o[0]]))
                         cops.append(ref(o))
-                L.append("      " + c_operation(opcode, types, cops))
-        L.append("    }")
-        for b, r in accs.items():
-            L.append("    %s[%d] = acc%d;" % (gen.arr(b), r[2][0][1], b))
+                body.append(c_operation(opcode, types, cops))
+        # the loop nest: parallel outermost loop, sequential middle loops, simd innermost loop with
+        # the scalar accumulators of its sweeps
+        ind = "    "
+        L.append("    #pragma omp " + pragma)
+        lines = []
+        for a in range(n):
+            if a == n - 1 and n > 1:
+                for r in inner_sweeps:
+                    t = np.dtype(self.dtypes[r[2][0][0]])
+                    lines.append(ind + "  " * a + "%s r%d = %s;" % (CTYPE[t.str], r[2][0][0],
+                                                                   c_literal(t, sweep_identity(r[1], t))))
+                if simd:
+                    red = "".join(" reduction(%s:r%d)" % (_OMP_RED[r[1]], r[2][0][0]) for r in inner_sweeps)
+                    lines.append(ind + "  " * a + "#pragma omp simd" + red)
+            lines.append(ind + "  " * a + "for (int64_t %s = 0; %s < %d; ++%s) {" % (names[a], names[a], S[a],
+                                                                                  names[a]))
+        lines.extend(ind + "  " * n + b for b in body)
+        for a in reversed(range(n)):
+            lines.append(ind + "  " * a + "}")
+            if a == n - 1 and n > 1:
+                for r in inner_sweeps:
+                    lines.append(ind + "  " * a + "%s = r%d;" % (out_ref(r, names[:-1]), r[2][0][0]))
+        L.extend(lines)
+        for r in clause:
+            o = r[2][0]
+            L.append("    %s[%s] = acc%d;" % (gen.arr(o[0]), _index(o, []), o[0]))
         L.append("  }")
         return L
 
@@ -625,8 +731,11 @@ class FusedProgram:
         import time
         recs, nrep, cond = self.groups[self._gi]
         for rec in recs:
-            if rec[0] == "new":
-                self.data[rec[1]] = np.zeros(max(1, rec[3]), dtype=np.dtype(rec[2]))
+            if rec[0] == "new" and rec[1] in self.arrays[self._gi]:
+                # zero-filled here, outside the timed region (first-touch page faults included)
+                a = np.empty(max(1, rec[3]), dtype=np.dtype(rec[2]))
+                a.fill(0)
+                self.data[rec[1]] = a
             elif rec[0] == "data":
                 self.data[rec[1]] = np.array(rec[2], dtype=np.dtype(self.dtypes[rec[1]])).reshape(-1)
         maxbid = max(self.dtypes) + 1 if self.dtypes else 1
@@ -643,4 +752,14 @@ class FusedProgram:
                 break
         dt = time.perf_counter() - t0
         self._gi += 1
+        for rec in recs:
+            if rec[0] == "free":
+                self.data.pop(rec[1], None)
         return dt
+
+    def run_all(self):
+        """Execute every group; returns (total seconds, {base id: data} of the bases alive at the end)."""
+        total = 0.0
+        while self._gi < len(self.groups):
+            total += self.run_next_group()
+        return total, dict(self.data)

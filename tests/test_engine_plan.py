@@ -137,3 +137,29 @@ def test_light_reductions_take_half_the_workgroup_cap():
     out = dict(l.split(" ", 1) for l in r.stdout.splitlines() if l.startswith(("JACOBI", "BS")))
     assert json.loads(out["JACOBI"]) == [2048]
     assert json.loads(out["BS"]) == [4096]
+
+
+def test_full_reduce_main_loop_is_the_first_sweeping_loop():
+    """With small sweeping tail blocks after it, a FULL_REDUCE launch is sized by its main loop
+    (5000 elements: 5 workgroups of 256 threads x 4 elements), not by the last tail loop."""
+    prog = r"""
+import sys
+sys.path.insert(0, %(root)r)
+import numpy as np
+import bohrium_amd as bh
+from bohrium_amd import workloads, _bh_api as api
+api.set_flush_threshold(10 ** 7)
+bh.message("fuser_list: reshapable_first, push_reductions_inwards, collapse_redundant_axes")
+sizes = np.array([5000, 200, 3000, 100])
+xs, idxs = workloads.batched_inputs(sizes)
+bh.flush()
+bh.message("plan log")
+res = workloads.batched_step(xs, idxs)
+bh.flush()
+print(bh.message("plan log"))
+""" % {"root": ROOT}
+    r = subprocess.run([sys.executable, "-c", prog], env=dict(os.environ, BH_HIP_DRY_RUN="1"),
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
+    grids = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("reduce ")]
+    assert grids and 5 in grids and 1 not in grids, r.stdout
