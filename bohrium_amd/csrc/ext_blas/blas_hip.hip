@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -497,17 +498,22 @@ template <Fill F> int fill(int dtype, const void *A, int64_t lda, void *out, int
     return launch_status();
 }
 
-// C = F(A) B with F(A) expanded from A's upper triangle into an m x m temporary.
+// C = F(A) B with F(A) expanded from A's upper triangle into an m x m buffer: a stream-ordered
+// temporary, or `staging` when the caller supplies one of at least m * m elements.
 template <Fill F>
 int expanded_product(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb,
-                     void *C, int64_t ldc, void *stream) {
+                     void *C, int64_t ldc, void *stream, void *staging = nullptr) {
     if (!check_call(dtype, m, n, m)) return 1;
     if (m == 0 || n == 0) return 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     DeviceTemp full(s);
-    if (!full.alloc(static_cast<size_t>(m * m * elem_size(dtype)))) return 1;
-    if (fill<F>(dtype, A, lda, full.p, m, s) != 0) return 1;
-    return dispatch<false, false, false>(make_gemm(dtype, m, n, m, full.p, m, B, ldb, C, ldc, stream));
+    void *expanded = staging;
+    if (expanded == nullptr) {
+        if (!full.alloc(static_cast<size_t>(m * m * elem_size(dtype)))) return 1;
+        expanded = full.p;
+    }
+    if (fill<F>(dtype, A, lda, expanded, m, s) != 0) return 1;
+    return dispatch<false, false, false>(make_gemm(dtype, m, n, m, expanded, m, B, ldb, C, ldc, stream));
 }
 
 template <typename T, int NB>
@@ -624,6 +630,44 @@ public:
 
 } // namespace
 
+// Staging area of bhblas_symm_noalloc: the expanded m x m matrix lives here instead of in a
+// temporary.  Every device that runs the library holds its own copy (at its own address) until
+// the process ends.  In host code `scratch` names a host-side stand-in, never the device memory:
+// host code uses only the address that hipGetSymbolAddress returned for the current device.
+static constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int scratch_address(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        g_err = "bhblas: no usable current device for the staging area";
+        return 1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            g_err = std::string("bhblas: hipGetSymbolAddress of the staging area failed: ") + hipGetErrorString(e);
+            return 1;
+        }
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return 0;
+}
+
+} // namespace
+
 extern "C" {
 
 int bhblas_gemm(int dtype, int trans_a, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, const void *B,
@@ -665,6 +709,21 @@ int bhblas_her2k(int dtype, int64_t n, int64_t k, const void *A, int64_t lda, co
 int bhblas_symm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
                 int64_t ldc, void *stream) {
     return expanded_product<Fill::SYM>(dtype, m, n, A, lda, B, ldb, C, ldc, stream);
+}
+
+int bhblas_symm_noalloc(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb,
+                        void *C, int64_t ldc, void *stream) {
+    if (!check_call(dtype, m, n, m)) return 1;
+    if (m == 0 || n == 0) return 0;
+    const uint64_t need = static_cast<uint64_t>(m) * static_cast<uint64_t>(m) * static_cast<uint64_t>(elem_size(dtype));
+    if (need > kScratchBytes) {
+        g_err = "bhblas: symm_noalloc needs m * m * element size <= " + std::to_string(kScratchBytes) +
+                " bytes of staging, got " + std::to_string(need) + " (m = " + std::to_string(m) + ")";
+        return 1;
+    }
+    void *staging = nullptr;
+    if (scratch_address(&staging) != 0) return 1;
+    return expanded_product<Fill::SYM>(dtype, m, n, A, lda, B, ldb, C, ldc, stream, staging);
 }
 
 int bhblas_hemm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,

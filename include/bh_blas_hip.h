@@ -70,6 +70,15 @@ int bhblas_her2k(int dtype, int64_t n, int64_t k, const void *A, int64_t lda, co
 /* C = A B, A m x m symmetric given by its upper triangle, B and C m x n (cblas_?symm, left, upper). */
 int bhblas_symm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
                 int64_t ldc, void *stream);
+/* bhblas_symm without any allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and result as bhblas_symm; the expanded m x m matrix is staged in a
+ * fixed per-device area of 32 MiB (33554432 bytes) that the library holds until the process ends, so
+ * m * m * element size must not exceed 33554432 (else it returns non-zero and says why).  The stream
+ * must belong to the current device.  Calls on one device share that area: calls that overlap on one
+ * device (two streams, or two host threads) race and must be ordered by the caller. */
+int bhblas_symm_noalloc(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb,
+                        void *C, int64_t ldc, void *stream);
 /* C = A B, A m x m Hermitian given by its upper triangle; complex only (cblas_?hemm, left, upper). */
 int bhblas_hemm(int dtype, int64_t m, int64_t n, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
                 int64_t ldc, void *stream);

@@ -17,6 +17,7 @@ HEADER_LIBS = {
     "bh_ve_hip.h": "libbh_ve_hip.so",
     "bh_costmodel.h": "libbh_ve_hip.so",
     "bh_mem_signal.h": "libbh.so",
+    "bh_blas_hip.h": "libbh_blas_hip.so",
 }
 
 
@@ -43,7 +44,8 @@ def test_every_declared_symbol_is_exported(header):
     assert not missing, "%s declares symbols %s missing from %s" % (header, missing, HEADER_LIBS[header])
 
 
-@pytest.mark.parametrize("lib", ["libbh.so", "libbhc.so", "libbh_vem_node.so", "libbh_ve_hip.so"])
+@pytest.mark.parametrize("lib", ["libbh.so", "libbhc.so", "libbh_vem_node.so", "libbh_ve_hip.so",
+                                 "libbh_blas_hip.so"])
 def test_libraries_load(lib):
     ctypes.CDLL(os.path.join(LIB, lib), mode=ctypes.RTLD_GLOBAL)
 
