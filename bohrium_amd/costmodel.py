@@ -20,6 +20,9 @@ FEATURES = ["log2_elements", "log2_bytes_in", "log2_bytes_out", "log2_bytes_temp
             "reductions", "accumulates", "gathers_scatters", "max_ndim", "strided_inputs", "inner_axis_reduce",
             "outer_axis_reduce", "full_reduce", "input_arrays", "fp64_fraction"]
 NFEAT = len(FEATURES)
+# The launch-shape model reads the block features followed by these (include/bh_costmodel.h).
+LAUNCH_FEATURES = ["log2_workgroups", "log2_issued_together", "log2_elements_per_thread"]
+NFEAT_LAUNCH = NFEAT + len(LAUNCH_FEATURES)
 
 _LIB = None
 
@@ -44,6 +47,7 @@ def lib():
         L.bhcm_gradient.argtypes = [vp, f32p, f32p, i64, f32p, f32p]
         L.bhcm_train.argtypes = [vp, f32p, f32p, i64, ctypes.c_int, ctypes.c_float, f32p]
         L.bhcm_install.argtypes = [vp]
+        L.bhcm_install_launch.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -122,6 +126,12 @@ class CostModel:
         _check(lib().bhcm_install(self.handle))
         _bh_api.message("plans: drop")
 
+    def install_launch(self):
+        """Use this model (NFEAT_LAUNCH features) as the autotuner's launch-shape model."""
+        assert self.nfeat == NFEAT_LAUNCH
+        _check(lib().bhcm_install_launch(self.handle))
+        _bh_api.message("plans: drop")
+
     def save(self, path):
         np.savez(path, params=self.params(), mean=self.mean, inv_std=self.inv_std,
                  shape=np.array([self.nfeat, self.hidden]))
@@ -138,6 +148,38 @@ class CostModel:
 def uninstall():
     _check(lib().bhcm_install(None))
     _bh_api.message("plans: drop")
+
+
+def uninstall_launch():
+    _check(lib().bhcm_install_launch(None))
+    _bh_api.message("plans: drop")
+
+
+def tuning_samples(text):
+    """(features n x NFEAT_LAUNCH, microseconds n) from the engine's "autotune samples" text."""
+    X, y = [], []
+    for line in text.splitlines():
+        parts = line.split()
+        if len(parts) != 2 or not parts[0].startswith("f=") or not parts[1].startswith("ms="):
+            continue
+        X.append([float(v) for v in parts[0][2:].split(",")])
+        y.append(float(parts[1][3:]) * 1e3)
+    if not X:
+        return np.zeros((0, NFEAT_LAUNCH), np.float32), np.zeros(0, np.float32)
+    return np.array(X, np.float32), np.array(y, np.float32)
+
+
+def tuning_outcomes(text):
+    """[(signature, chosen, measured ms per candidate, predicted ms per candidate or None)] from the
+    engine's "autotune outcomes" text (candidate 0: the configured knobs)."""
+    out = []
+    for line in text.splitlines():
+        f = dict(p.split("=", 1) for p in line.split() if "=" in p)
+        if "ms" not in f:
+            continue
+        pred = None if f["predicted_ms"] == "none" else [float(v) for v in f["predicted_ms"].split(",")]
+        out.append((f["key"], int(f["chosen"]), [float(v) for v in f["ms"].split(",")], pred))
+    return out
 
 
 def profiled_kernels(launch_text):

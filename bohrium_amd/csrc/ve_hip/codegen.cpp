@@ -1369,7 +1369,7 @@ void Writer::emit_scan() {
     }
     body << "    }\n";
     body << "    x_ = op_(carry_, bh_wave_inclusive_scan(x_, op_));\n";
-    body << "    carry_ = bh_shfl<" << T << ">::idx(x_, 63);\n";
+    body << "    carry_ = bh_lane(x_, 63);\n";
     body << "    v_[j_] = x_;\n  }\n";
     body << "  if (lane_ == 63) bh_wtot_[wave_] = carry_;\n  __syncthreads();\n";
     body << "  " << T << " wpre_ = " << ident << ", ttot_ = " << ident << ";\n";
@@ -1476,9 +1476,13 @@ KernelPlan Writer::plan(Strategy s) {
     kp.attributes = cfg.waves_per_eu > 0 && s != Strategy::SERIAL
                         ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(cfg.waves_per_eu) + ")))"
                         : "";
+    // With xcd_remap the grid-stride skeletons number their workgroups per XCD (bh_xcd_block); the
+    // scan skeleton takes its tiles from a ticket counter and the serial one has one workgroup.
+    const bool remap = cfg.xcd_remap && (s == Strategy::MAP || s == Strategy::FULL_REDUCE || s == Strategy::ROW_REDUCE);
     src << kp.device_code << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ")" << kp.attributes
         << " " << kp.name << "("
-        << plist << ") {\n  bh_main_(" << nlist << (nlist.empty() ? "" : ", ") << "blockIdx.x, gridDim.x);\n}\n";
+        << plist << ") {\n  bh_main_(" << nlist << (nlist.empty() ? "" : ", ")
+        << (remap ? "bh_xcd_block(blockIdx.x, gridDim.x)" : "blockIdx.x") << ", gridDim.x);\n}\n";
     if (s == Strategy::FULL_REDUCE)
         src << "\nextern \"C\" __global__ void __launch_bounds__(" << kp.block << ") " << kp.fin_name << "(" << plist
             << ") {\n  bh_fin_(" << nlist << ");\n}\n";
@@ -1777,6 +1781,64 @@ std::vector<size_t> arg_offsets(const std::vector<ArgSpec> &args, size_t *total)
     return off;
 }
 
+std::string chain_source(const std::vector<const KernelPlan *> &stages, std::vector<size_t> &offsets,
+                         size_t *stride) {
+    std::ostringstream s;
+    offsets.clear();
+    size_t at = 0;
+    for (size_t k = 0; k < stages.size(); ++k) {
+        const KernelPlan &p = *stages[k];
+        const std::string tag = "_s" + std::to_string(k) + "_";
+        std::string code = p.device_code;
+        for (const char *fn : {"bh_main_(", "bh_fin_("}) {
+            const std::string from = std::string("void ") + fn;
+            const std::string to = std::string("void ") + std::string(fn).substr(0, std::string(fn).size() - 2) + tag + "(";
+            for (size_t pos = code.find(from); pos != std::string::npos; pos = code.find(from, pos + to.size()))
+                code.replace(pos, from.size(), to);
+        }
+        s << code << "\n";
+        offsets.push_back(at);
+        size_t packed = 0;
+        arg_offsets(p.args, &packed);
+        at += (packed + 15) / 16 * 16 + 16;  // the stage's packed arguments, then its launch grid (uint32)
+    }
+    *stride = at;
+    // One workgroup per item (items ordered by the host, largest first); the stages run one after the
+    // other with a workgroup barrier between them: every stage with one workgroup (a full reduction
+    // leaves one partial, which its finalize folds; a scan takes its tiles in order).
+    s << "\nextern \"C\" __global__ void __launch_bounds__(256) bh_kernel_chain("
+      << "const unsigned char *__restrict__ bh_args, const unsigned *__restrict__ bh_order, const unsigned bh_stride) {\n"
+      << "  const unsigned char *I_ = bh_args + (size_t)bh_order[blockIdx.x] * bh_stride;\n";
+    for (size_t k = 0; k < stages.size(); ++k) {
+        const KernelPlan &p = *stages[k];
+        const std::vector<size_t> off = arg_offsets(p.args, nullptr);
+        std::string unpack;
+        for (size_t i = 0; i < p.param_types.size(); ++i) {
+            const std::string &t = p.param_types[i];
+            const bool ptr = !t.empty() && t.back() == '*';
+            unpack += std::string(i ? ",\n      " : "") + "*(" + t + (ptr ? "const *" : " const *") + ")(A_ + " +
+                      std::to_string(off[i]) + ")";
+        }
+        const std::string sep = unpack.empty() ? "" : ", ";
+        const std::string tag = "_s" + std::to_string(k) + "_";
+        size_t packed = 0;
+        arg_offsets(p.args, &packed);
+        s << "  {\n    const unsigned char *A_ = I_ + " << offsets[k] << ";\n";
+        if (p.strategy == Strategy::SCAN) {
+            // The item's tiles in order: each call takes the next ticket, whose predecessors are done.
+            s << "    const unsigned nt_ = *(const unsigned *)(A_ + " << (packed + 15) / 16 * 16 << ");\n"
+              << "    for (unsigned t_ = 0; t_ < nt_; ++t_) {\n      bh_main" << tag << "(" << unpack << sep
+              << "t_, nt_);\n      __syncthreads();\n    }\n";
+        } else {
+            s << "    bh_main" << tag << "(" << unpack << sep << "0u, 1u);\n    __syncthreads();\n";
+            if (!p.fin_name.empty()) s << "    bh_fin" << tag << "(" << unpack << ");\n    __syncthreads();\n";
+        }
+        s << "  }\n";
+    }
+    s << "}\n";
+    return s.str();
+}
+
 std::string batch_source(const KernelPlan &p) {
     const std::vector<size_t> off = arg_offsets(p.args, nullptr);
     std::string unpack;
@@ -1794,11 +1856,12 @@ std::string batch_source(const KernelPlan &p) {
     s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ")" << p.attributes << " bh_kernel_batch("
       << "const unsigned char *__restrict__ bh_args, const unsigned *__restrict__ bh_first, const unsigned bh_items, "
          "const unsigned bh_stride) {\n"
+      << "  const unsigned b_ = blockIdx.x;\n"
       << "  unsigned lo_ = 0, hi_ = bh_items;\n"
-      << "  while (hi_ - lo_ > 1u) { const unsigned mid_ = (lo_ + hi_) >> 1; if (bh_first[mid_] <= blockIdx.x) lo_ = mid_; "
+      << "  while (hi_ - lo_ > 1u) { const unsigned mid_ = (lo_ + hi_) >> 1; if (bh_first[mid_] <= b_) lo_ = mid_; "
          "else hi_ = mid_; }\n"
       << "  const unsigned char *A_ = bh_args + (size_t)lo_ * bh_stride;\n"
-      << "  bh_main_(" << unpack << sep << "blockIdx.x - bh_first[lo_], bh_first[lo_ + 1] - bh_first[lo_]);\n}\n";
+      << "  bh_main_(" << unpack << sep << "b_ - bh_first[lo_], bh_first[lo_ + 1] - bh_first[lo_]);\n}\n";
     if (!p.fin_name.empty()) {
         s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ") bh_kernel_fin_batch("
           << "const unsigned char *__restrict__ bh_args, const unsigned bh_stride) {\n"

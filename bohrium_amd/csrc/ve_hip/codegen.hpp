@@ -103,6 +103,8 @@ struct WriterConfig {
     bool resident_grid = false;
     /* float64 exp and log through the prelude's bh_exp64 / bh_log64 (false: the HIP math library's). */
     bool math64 = true;
+    /* Grid-stride skeletons renumber their workgroups so that each XCD runs a contiguous range. */
+    bool xcd_remap = false;
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
 
@@ -131,6 +133,13 @@ std::vector<size_t> arg_offsets(const std::vector<ArgSpec> &args, size_t *total)
  * code, each with its own packed arguments and workgroup count (and bh_kernel_fin_batch, one
  * workgroup per item, for FULL_REDUCE). */
 std::string batch_source(const KernelPlan &plan);
+
+/* Source of the chained form of a sequence of plans (stages): bh_kernel_chain runs one item per
+ * workgroup of 256 threads, all stages of the item in order with a barrier between them, each stage
+ * as one workgroup of its skeleton (MAP, FULL_REDUCE with its finalize, SCAN tile by tile).  An item's
+ * arguments are the stages' packed arguments at `offsets` (16-byte aligned), each followed by 16
+ * bytes holding the stage's launch grid (uint32); `stride` is the item's total. */
+std::string chain_source(const std::vector<const KernelPlan *> &stages, std::vector<size_t> &offsets, size_t *stride);
 
 /* HIP spelling of an element type inside generated kernels. */
 const char *hip_type(bh_type t);

@@ -211,6 +211,7 @@ struct bhcm_model {
 
 namespace {
 bhcm_model *g_installed = nullptr;
+bhcm_model *g_installed_launch = nullptr;  // the launch-shape model (block and launch features)
 std::mutex g_install_mu;
 
 int reserve(bhcm_model *m, int64_t n) {
@@ -303,6 +304,7 @@ int bhcm_destroy(bhcm_model *m) {
     {
         std::lock_guard<std::mutex> lock(g_install_mu);
         if (g_installed == m) g_installed = nullptr;
+        if (g_installed_launch == m) g_installed_launch = nullptr;
     }
     (void)hipDeviceSynchronize();
     float *bufs[] = {m->P, m->M, m->V, m->G, m->mean, m->istd, m->loss, m->X, m->T, m->Y, m->H1, m->H2, m->DY, m->DH1,
@@ -392,6 +394,12 @@ int bhcm_install(bhcm_model *m) {
     return 0;
 }
 
+int bhcm_install_launch(bhcm_model *m) {
+    std::lock_guard<std::mutex> lock(g_install_mu);
+    g_installed_launch = m;
+    return 0;
+}
+
 } // extern "C"
 
 namespace bohrium {
@@ -409,6 +417,20 @@ bool costmodel_predict(const std::vector<float> &x, int64_t n, std::vector<float
 int costmodel_features() {
     std::lock_guard<std::mutex> lock(g_install_mu);
     return g_installed == nullptr ? 0 : g_installed->L.F;
+}
+
+bool launchmodel_predict(const std::vector<float> &x, int64_t n, std::vector<float> &y) {
+    std::lock_guard<std::mutex> lock(g_install_mu);
+    if (g_installed_launch == nullptr) return false;
+    y.resize(static_cast<size_t>(n));
+    if (n == 0) return true;
+    if (static_cast<int64_t>(x.size()) != n * g_installed_launch->L.F) return false;
+    return bhcm_predict(g_installed_launch, x.data(), n, y.data()) == 0;
+}
+
+int launchmodel_features() {
+    std::lock_guard<std::mutex> lock(g_install_mu);
+    return g_installed_launch == nullptr ? 0 : g_installed_launch->L.F;
 }
 } // namespace vehip
 } // namespace bohrium

@@ -156,6 +156,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     wcfg.reduce_unroll = std::max(1, cfg.defaultGet<int>("reduce_unroll", 1));
     wcfg.row_reduce = cfg.defaultGet<bool>("row_reduce", true);
     wcfg.waves_per_eu = std::max(0, cfg.defaultGet<int>("waves_per_eu", 0));
+    wcfg.xcd_remap = cfg.defaultGet<bool>("xcd_remap", false);
     unflatten = cfg.defaultGet<bool>("unflatten_reductions", true);
     resident = cfg.defaultGet<bool>("resident", true);
     track_host_writes = cfg.defaultGet<bool>("host_write_tracking", true);
@@ -172,6 +173,8 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     tuner.samples = std::max(1, cfg.defaultGet<int>("autotune_samples", 3));
     tuner.min_items = cfg.defaultGet<uint64_t>("autotune_min_items", 1u << 20);
     tuner.min_gain = cfg.defaultGet<double>("autotune_min_gain", 0.02);
+    tuner.mode = cfg.defaultGet<std::string>("autotune_mode", "measure") == "model" ? Tuner::Mode::MODEL
+                                                                                    : Tuner::Mode::MEASURE;
     if (!cache_dir.empty() && cfg.defaultGet<bool>("autotune_persist", true)) {
         tuner.path = cache_dir + "/autotune_" + cfg.defaultGet<std::string>("dry_run_arch", "gfx950") + ".txt";
         tuner.load();
@@ -639,11 +642,11 @@ void *EngineHip::graphFor(const std::vector<IssuedLaunch> &launches) {
     return exec;
 }
 
-void EngineHip::graphLaunch(void *graph, size_t nlaunches) {
+void EngineHip::graphLaunch(void *graph, size_t nlaunches, uint64_t iterations) {
     check(bhhip_graph_launch(graph), "graph launch");
-    ++graph_replays;
+    graph_replays += iterations;
     stats.launches += nlaunches;
-    ++stats.flushes;
+    stats.flushes += iterations;
 }
 
 void EngineHip::checkErrors() {
@@ -848,6 +851,28 @@ void EngineHip::tuneKernel(CachedKernel &ck, const jitk::LoopB &kernel,
         t->variants.push_back(std::move(plans));
     }
     if (t->variants.size() < 2) return;
+    // Block + launch features of each candidate, and the launch-shape model's predictions.
+    const std::vector<float> block = ck.plans.front().features;
+    for (size_t c = 0; c < t->variants.size(); ++c) {
+        const LaunchKnobs &k = t->knobs[c];
+        const int per_thread = big->strategy == Strategy::MAP           ? k.map_unroll
+                               : big->strategy == Strategy::FULL_REDUCE ? k.reduce_unroll
+                               : big->strategy == Strategy::SCAN        ? k.scan_items
+                                                                        : 1;
+        std::vector<float> row = block;
+        const std::vector<float> lf = launch_features(t->variants[c], per_thread);
+        row.insert(row.end(), lf.begin(), lf.end());
+        t->features.push_back(std::move(row));
+    }
+    if (!predict_launch_times(t->features, t->predicted_ms)) t->predicted_ms.clear();
+    if (tuner.mode == Tuner::Mode::MODEL && !t->predicted_ms.empty()) {
+        size_t pick = 0;
+        for (size_t c = 1; c < t->predicted_ms.size(); ++c)
+            if (t->predicted_ms[c] < t->predicted_ms[pick]) pick = c;
+        ck.plans = t->variants[pick];
+        ++tuner.model_picks;
+        return;
+    }
     t->ms.resize(t->variants.size());
     t->inflight.assign(t->variants.size(), 0);
     ck.trial = std::move(t);
@@ -866,6 +891,8 @@ void EngineHip::pollTrial(Trial &t, bool wait) {
         float ms = 0;
         check(bhhip_event_elapsed_ms(p.ev0, p.ev1, &ms), "event timing");
         t.ms[p.cand].push_back(ms);
+        if (tuner.samples_log.size() < tuner.log_limit && p.cand < t.features.size())
+            tuner.samples_log.push_back({t.features[p.cand], static_cast<double>(ms)});
         --t.inflight[p.cand];
         bhhip_event_destroy(p.ev0);
         bhhip_event_destroy(p.ev1);
@@ -895,6 +922,7 @@ bool EngineHip::settleTrial(CachedKernel &ck) {
         if (best[c] < best[pick]) pick = c;
     if (pick != 0 && best[pick] > best[0] * (1.0 - tuner.min_gain)) pick = 0;
     if (pick == 0) ++tuner.kept_default;
+    if (tuner.outcomes_log.size() < tuner.log_limit) tuner.outcomes_log.push_back({t.key, best, t.predicted_ms, pick});
     ++tuner.decisions;
     tuner.saved_ms += best[0] - best[pick];
     Tuner::Entry e;
@@ -1152,6 +1180,7 @@ std::string EngineHip::statistics() const {
     ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";
     ss << "twin_exchanges: " << twin_exchanges << "\ntwin_full_copies: " << twin_full_copies << "\n";
+    ss << "autotune_model_picks: " << tuner.model_picks << "\n";
     ss << "autotune_trials: " << tuner.trials << "\nautotune_decisions: " << tuner.decisions
        << "\nautotune_kept_configured: " << tuner.kept_default << "\nautotune_table_hits: " << tuner.table_hits
        << "\nautotune_saved_ms: " << tuner.saved_ms << "\n";
@@ -1204,6 +1233,9 @@ void slide_views(BhIR *bhir) {
     }
 }
 
+// Periods of a replayed repeat body captured in one graph.
+constexpr uint64_t kGraphRounds = 8;
+
 class Impl : public component::ComponentVE {
 public:
     EngineHip engine;
@@ -1245,13 +1277,37 @@ public:
             EngineHip::FlushEffects effects;
         };
         std::deque<Iteration> hist;
+        std::vector<EngineHip::IssuedLaunch> phase_launches[2];
         std::vector<EngineHip::IssuedLaunch> cur;
         void *graphs[2] = {nullptr, nullptr};
         size_t graph_launches[2] = {0, 0};
         EngineHip::FlushEffects graph_effects[2];
         int period = 0, phase = 0;
+        // Several periods in one graph: fewer graph launches (each costs a submission and a gap on
+        // the device); `rounds` periods per launch of `multi`.
+        void *multi = nullptr;
+        uint64_t rounds = 0;
+        size_t multi_launches = 0;
         for (uint64_t i = 0; i < nrepeats; ++i) {
             if (period > 0) {
+                if (multi == nullptr && phase == 0 && nrepeats - i >= 2 * static_cast<uint64_t>(period)) {
+                    rounds = std::min<uint64_t>(kGraphRounds, (nrepeats - i) / static_cast<uint64_t>(period));
+                    std::vector<EngineHip::IssuedLaunch> seq;
+                    for (uint64_t r = 0; r < rounds; ++r)
+                        for (int ph = 0; ph < period; ++ph) {
+                            const std::vector<EngineHip::IssuedLaunch> &l = phase_launches[ph];
+                            seq.insert(seq.end(), l.begin(), l.end());
+                        }
+                    multi = engine.graphFor(seq);
+                    multi_launches = seq.size();
+                }
+                if (multi != nullptr && phase == 0 && nrepeats - i >= rounds * static_cast<uint64_t>(period)) {
+                    engine.graphLaunch(multi, multi_launches, rounds * static_cast<uint64_t>(period));
+                    for (uint64_t r = 0; r < rounds; ++r)
+                        for (int ph = 0; ph < period; ++ph) engine.applyEffects(graph_effects[ph]);
+                    i += rounds * static_cast<uint64_t>(period) - 1;
+                    continue;
+                }
                 engine.graphLaunch(graphs[phase], graph_launches[phase]);
                 engine.applyEffects(graph_effects[phase]);
                 phase = (phase + 1) % period;
@@ -1283,6 +1339,7 @@ public:
                             graphs[0] = engine.graphFor(last.launches);
                             graph_launches[0] = last.launches.size();
                             graph_effects[0] = last.effects;
+                            phase_launches[0] = last.launches;
                             period = 1;
                         }
                     } else if (n >= 3 && last.launches == hist[n - 3].launches) {
@@ -1296,6 +1353,8 @@ public:
                             graphs[1] = engine.graphFor(last.launches);
                             graph_launches[1] = last.launches.size();
                             graph_effects[1] = last.effects;
+                            phase_launches[0] = hist[n - 2].launches;
+                            phase_launches[1] = last.launches;
                             period = 2;
                         }
                     } else {
@@ -1355,7 +1414,7 @@ public:
                 const std::string k = kv.substr(0, eq);
                 const int v = std::stoi(kv.substr(eq + 1));
                 if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu" && k != "resident_grid" &&
-                              k != "math64"))
+                              k != "math64" && k != "xcd_remap"))
                     throw std::invalid_argument("hip engine: codegen setting must be >= 1: " + kv);
                 if (k == "max_grid") {
                     engine.codegenConfig().max_grid = v;
@@ -1367,6 +1426,8 @@ public:
                     engine.codegenConfig().resident_grid = v != 0;
                 } else if (k == "math64") {
                     engine.codegenConfig().math64 = v != 0;
+                } else if (k == "xcd_remap") {
+                    engine.codegenConfig().xcd_remap = v != 0;
                 } else if (k == "reduce_max_grid") {
                     engine.codegenConfig().reduce_max_grid = v;
                 } else if (k == "reduce_min_items") {
@@ -1386,7 +1447,7 @@ public:
                << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items
                << " scan_items=" << w.scan_items << " row_reduce=" << (w.row_reduce ? 1 : 0)
                << " waves_per_eu=" << w.waves_per_eu << " resident_grid=" << (w.resident_grid ? 1 : 0)
-               << " math64=" << (w.math64 ? 1 : 0);
+               << " math64=" << (w.math64 ? 1 : 0) << " xcd_remap=" << (w.xcd_remap ? 1 : 0);
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
@@ -1405,6 +1466,28 @@ public:
         } else if (msg.rfind("autotune_min_items:", 0) == 0) {
             engine.tuner.min_items = std::stoull(msg.substr(19));
             engine.dropPlans();
+        } else if (msg == "autotune_mode: measure" || msg == "autotune_mode: model") {
+            engine.tuner.mode = msg == "autotune_mode: model" ? Tuner::Mode::MODEL : Tuner::Mode::MEASURE;
+            engine.dropPlans();
+        } else if (msg == "autotune samples") {
+            // Timed trial launches since the last read: "f=<block and launch features> ms=<device ms>".
+            for (const Tuner::Sample &x : engine.tuner.samples_log) {
+                ss << "f=";
+                for (size_t k = 0; k < x.features.size(); ++k) ss << (k ? "," : "") << x.features[k];
+                ss << " ms=" << x.ms << "\n";
+            }
+            engine.tuner.samples_log.clear();
+        } else if (msg == "autotune outcomes") {
+            // Settled trials since the last read: every candidate's best time, the model's predictions.
+            for (const Tuner::Outcome &o : engine.tuner.outcomes_log) {
+                ss << "key=" << o.key << " chosen=" << o.chosen << " ms=";
+                for (size_t k = 0; k < o.best_ms.size(); ++k) ss << (k ? "," : "") << o.best_ms[k];
+                ss << " predicted_ms=";
+                if (o.predicted_ms.empty()) ss << "none";
+                for (size_t k = 0; k < o.predicted_ms.size(); ++k) ss << (k ? "," : "") << o.predicted_ms[k] * 1e-3;
+                ss << "\n";
+            }
+            engine.tuner.outcomes_log.clear();
         } else if (msg == "autotune") {
             // Settle what can be settled (synchronizes), then list the table.
             engine.resolveTuning();

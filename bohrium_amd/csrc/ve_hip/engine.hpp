@@ -125,7 +125,8 @@ public:
     std::vector<uint64_t> quietToken() const;
     /* A graph holding `launches` in order (captured once per distinct sequence, then cached). */
     void *graphFor(const std::vector<IssuedLaunch> &launches);
-    void graphLaunch(void *graph, size_t nlaunches);
+    /* Launch a captured graph that stands for `iterations` repeat iterations (for the statistics). */
+    void graphLaunch(void *graph, size_t nlaunches, uint64_t iterations = 1);
     bool graph_repeats = true;
     uint64_t graph_replays = 0;
     uint64_t kernel_seq = 0;  // id of the jitk kernel whose plans are being launched
@@ -217,6 +218,8 @@ private:
         std::string key;
         std::vector<LaunchKnobs> knobs;  // [0]: the configured knobs
         std::vector<std::vector<KernelPlan> > variants;
+        std::vector<std::vector<float> > features;  // block + launch features per candidate
+        std::vector<double> predicted_ms;           // the launch-shape model's, when installed
         std::vector<std::vector<double> > ms;
         std::vector<int> inflight;
         struct Pending {

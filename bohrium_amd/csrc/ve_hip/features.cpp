@@ -1,5 +1,7 @@
 #include "features.hpp"
+#include "codegen.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <set>
@@ -12,6 +14,8 @@ namespace vehip {
 
 bool costmodel_predict(const std::vector<float> &x, int64_t n, std::vector<float> &y);  // costmodel.hip
 int costmodel_features();
+bool launchmodel_predict(const std::vector<float> &x, int64_t n, std::vector<float> &y);
+int launchmodel_features();
 
 const char *const kFeatureNames[kNumFeatures] = {
     "log2_elements",   "log2_bytes_in",  "log2_bytes_out",  "log2_bytes_temp", "log2_instrs",   "transcendentals",
@@ -126,6 +130,35 @@ bool predict_block_times(const std::vector<std::vector<jitk::InstrPtr> > &sets, 
     if (!costmodel_predict(x, static_cast<int64_t>(sets.size()), y)) return false;
     us.resize(sets.size());
     for (size_t k = 0; k < sets.size(); ++k) us[k] = std::exp(static_cast<double>(y[k]));
+    return true;
+}
+
+std::vector<float> launch_features(const std::vector<KernelPlan> &plans, int unroll_or_items) {
+    const KernelPlan *big = nullptr;
+    for (const KernelPlan &p : plans)
+        if (big == nullptr || p.work_items > big->work_items) big = &p;
+    std::vector<float> f(kNumLaunchFeatures, 0.0f);
+    if (big == nullptr) return f;
+    const double threads = static_cast<double>(std::max<uint64_t>(1, big->grid)) * static_cast<double>(big->block);
+    f[0] = static_cast<float>(std::log2(static_cast<double>(std::max<uint64_t>(1, big->grid))));
+    f[1] = static_cast<float>(std::log2(static_cast<double>(std::max(1, unroll_or_items))));
+    f[2] = static_cast<float>(std::log2(std::max(1.0, static_cast<double>(big->work_items) / threads)));
+    return f;
+}
+
+bool predict_launch_times(const std::vector<std::vector<float> > &rows, std::vector<double> &us) {
+    const int nf = kNumFeatures + kNumLaunchFeatures;
+    if (launchmodel_features() != nf) return false;
+    std::vector<float> x;
+    x.reserve(rows.size() * static_cast<size_t>(nf));
+    for (const auto &r : rows) {
+        if (static_cast<int>(r.size()) != nf) return false;
+        x.insert(x.end(), r.begin(), r.end());
+    }
+    std::vector<float> y;
+    if (!launchmodel_predict(x, static_cast<int64_t>(rows.size()), y)) return false;
+    us.resize(rows.size());
+    for (size_t k = 0; k < rows.size(); ++k) us[k] = std::exp(static_cast<double>(y[k]));
     return true;
 }
 

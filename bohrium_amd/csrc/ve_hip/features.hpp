@@ -31,5 +31,15 @@ std::string features_string(const std::vector<float> &f);
  * false when no model is installed. */
 bool predict_block_times(const std::vector<std::vector<jitk::InstrPtr> > &sets, std::vector<double> &us);
 
+/* Launch features of a kernel's plans under one knob set: log2 of the largest launch's workgroups,
+ * log2 of the elements a thread of it issues together (unroll, or scan items), log2 of its
+ * elements per thread (work items / (workgroups x workgroup size)). */
+constexpr int kNumLaunchFeatures = 3;
+std::vector<float> launch_features(const std::vector<struct KernelPlan> &plans, int unroll_or_items);
+
+/* Predicted device time (microseconds) of each row of block features + launch features with the
+ * installed launch-shape model; false when none is installed. */
+bool predict_launch_times(const std::vector<std::vector<float> > &rows, std::vector<double> &us);
+
 } // namespace vehip
 } // namespace bohrium

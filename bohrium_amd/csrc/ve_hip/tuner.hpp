@@ -44,6 +44,10 @@ struct LaunchKnobs {
 class Tuner {
 public:
     bool enabled = true;
+    /* measure: trials decide (the default); model: the launch-shape model (bhcm_install_launch)
+     * picks the candidate with the lowest predicted time and no trial runs (measure when no model
+     * is installed). */
+    enum class Mode { MEASURE, MODEL } mode = Mode::MEASURE;
     int samples = 3;                  // timed launches per candidate
     uint64_t min_items = 1u << 20;    // smaller kernels keep the configured knobs
     double min_gain = 0.02;           // a candidate must beat the configured knobs by this fraction
@@ -66,8 +70,24 @@ public:
     void clear() { table.clear(); }
     std::string dump() const;
 
-    uint64_t trials = 0, decisions = 0, table_hits = 0, kept_default = 0;
+    uint64_t trials = 0, decisions = 0, table_hits = 0, kept_default = 0, model_picks = 0;
     double saved_ms = 0;  // sum over decisions of (configured - chosen) per-launch device time
+
+    /* Training data of the launch-shape model: block + launch features of each timed trial launch
+     * and its device time; and per settled trial, every candidate's best time and (when a launch
+     * model is installed) its prediction.  Kept up to `log_limit` entries until read. */
+    struct Sample {
+        std::vector<float> features;
+        double ms;
+    };
+    struct Outcome {
+        std::string key;
+        std::vector<double> best_ms, predicted_ms;  // per candidate ([0]: configured knobs)
+        size_t chosen = 0;
+    };
+    std::vector<Sample> samples_log;
+    std::vector<Outcome> outcomes_log;
+    size_t log_limit = 200000;
 
 private:
     std::map<std::string, Entry> table;

@@ -211,6 +211,8 @@ if __name__ == "__main__":
     # Small drivers for profiling runs:
     #   python -m bohrium_amd.workloads jacobi [n] [sweeps]       (default 8192, 5)
     #   python -m bohrium_amd.workloads black_scholes [n] [steps] (default 20000000, 10)
+    #   python -m bohrium_amd.workloads mc_pi [n] [runs]          (default 10000000, 40)
+    #   python -m bohrium_amd.workloads batched [arrays] [runs]   (default 4096, 5)
     import sys
     what = sys.argv[1] if len(sys.argv) > 1 else "jacobi"
     if what == "jacobi":
@@ -223,5 +225,24 @@ if __name__ == "__main__":
         st = int(sys.argv[3]) if len(sys.argv) > 3 else 10
         S, K, T = black_scholes_inputs(n)
         print("black-scholes n=%d: sums %s" % (n, black_scholes_repeat(S, K, T, st).asnumpy()[:3]))
+    elif what == "mc_pi":
+        n = int(sys.argv[2]) if len(sys.argv) > 2 else 10_000_000
+        reps = int(sys.argv[3]) if len(sys.argv) > 3 else 40
+        for _ in range(reps):
+            c = int(mc_pi(n).asnumpy())
+        print("mc_pi n=%d: count %d" % (n, c))
+    elif what == "batched":
+        from . import _bh_api
+        count = int(sys.argv[2]) if len(sys.argv) > 2 else 4096
+        reps = int(sys.argv[3]) if len(sys.argv) > 3 else 5
+        sizes = batched_sizes(count)
+        _bh_api.set_flush_threshold(10 ** 7)
+        xs, idxs = batched_inputs(sizes)
+        _b.flush()
+        for _ in range(reps):
+            sums, cums, gathered = batched_step(xs, idxs)
+            _b.asnumpy_many([sums[-1], cums[-1][:1], gathered[-1][:1]])
+            del sums, cums, gathered
+        print("batched %d arrays (%d elements), %d runs" % (count, int(sizes.sum()), reps))
     else:
-        raise SystemExit("usage: python -m bohrium_amd.workloads jacobi|black_scholes [n] [steps]")
+        raise SystemExit("usage: python -m bohrium_amd.workloads jacobi|black_scholes|mc_pi|batched [n] [steps]")

@@ -47,6 +47,13 @@ int bhcm_gradient(bhcm_model *m, const float *x, const float *t, int64_t n, floa
 int bhcm_train(bhcm_model *m, const float *x, const float *t, int64_t n, int steps, float lr, float *loss);
 /* Make `m` the model the engine's learned fuser uses (NULL: none).  The caller keeps ownership. */
 int bhcm_install(bhcm_model *m);
+/* Make `m` the launch-shape model of the engine's autotuner (NULL: none): it predicts a kernel's
+ * log(us) from the 16 block features followed by 3 launch features (log2 of the largest launch's
+ * workgroups, log2 of the elements a thread issues together, log2 of the elements per thread).
+ * With autotune_mode = model the tuner launches the candidate with the lowest prediction and runs
+ * no trials; in every mode the tuner reports the prediction of each candidate it measures.  The
+ * caller keeps ownership. */
+int bhcm_install_launch(bhcm_model *m);
 
 #ifdef __cplusplus
 }
