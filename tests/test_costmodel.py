@@ -127,3 +127,61 @@ def test_learned_fuser_with_the_shipped_model():
         costmodel.uninstall()
         api.set_flush_threshold(1000)
         api.message("fuser_list: " + ct.PIPELINES["greedy"])
+
+
+@pytest.mark.gpu
+def test_launch_model_with_the_shipped_model():
+    """The launch-shape model trained on this device (bohrium_amd/data/launchmodel_mi355x.npz, made by
+    bohrium_amd.launchmodel_train from programs 100-107) against measurement on held-out programs:
+    the autotuner measures every candidate shape of each kernel and reports the model's prediction
+    of each, so the model's choice is judged on the same kernels against the measured-best
+    candidate (within 5 % in total) and against the configured knobs (at most 3 % slower in total: the
+    totals are sums of device times read from HIP events, which move by a few per cent between
+    runs)."""
+    from bohrium_amd import _bh_api as api
+    from bohrium_amd import costmodel
+    from bohrium_amd import launchmodel_train as lt
+    model = costmodel.CostModel.load(lt.DEFAULT_MODEL)
+    model.install_launch()
+    api.set_flush_threshold(10 ** 7)
+    try:
+        picks, tot_model, tot_best, tot_default = lt.judge([1003, 1005], sides=(1024, 2048), extra=False)
+        print("kernels %d model %.3f best %.3f configured %.3f" % (len(picks), tot_model, tot_best, tot_default))
+        assert len(picks) >= 10
+        assert tot_model <= 1.05 * tot_best
+        assert tot_model <= 1.03 * tot_default
+    finally:
+        costmodel.uninstall_launch()
+        api.set_flush_threshold(1000)
+        api.message("autotune: forget")
+
+
+@pytest.mark.gpu
+def test_model_mode_picks_without_trials_and_keeps_results():
+    """autotune_mode = model: kernels take the launch shape the model predicts fastest, no trial runs,
+    and the results are those of the oracle on the same bytecode (tolerance of the random-program
+    parity tests: rtol 1e-4 plus 1e-4 of the largest output magnitude, for float32 chains of a few
+    hundred operations with 1024-term sums reassociated differently on the device)."""
+    import bohrium_amd as bh
+    from bohrium_amd import _bh_api as api
+    from bohrium_amd import costmodel, workloads
+    from bohrium_amd import launchmodel_train as lt
+    from tests.helpers import run_both
+    model = costmodel.CostModel.load(lt.DEFAULT_MODEL)
+    model.install_launch()
+    api.set_flush_threshold(10 ** 7)
+    bh.message("autotune: forget")
+    bh.message("autotune_mode: model")
+    try:
+        t0, m0 = lt._stat("autotune_trials"), lt._stat("autotune_model_picks")
+        got, want = run_both(lambda: workloads.random_dag(1004, 120, sides=(1024,)))
+        assert lt._stat("autotune_model_picks") > m0
+        assert lt._stat("autotune_trials") == t0
+        scale = max([1.0] + [float(np.abs(w).max()) for w in want if w.size])
+        for g, w in zip(got, want):
+            np.testing.assert_allclose(g, w, rtol=1e-4, atol=1e-4 * scale)
+    finally:
+        bh.message("autotune_mode: measure")
+        costmodel.uninstall_launch()
+        api.set_flush_threshold(1000)
+        bh.message("autotune: forget")
