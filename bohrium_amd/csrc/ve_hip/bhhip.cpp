@@ -282,6 +282,14 @@ int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block
     return 0;
 }
 
+int bhhip_launch_cooperative(void *function, uint32_t grid, uint32_t block, void **params) {
+    if (grid == 0) return 0;
+    HIPCHK(hipModuleLaunchCooperativeKernel(reinterpret_cast<hipFunction_t>(function), grid, 1, 1, block, 1, 1, 0,
+                                            g_stream, params),
+           "hipModuleLaunchCooperativeKernel");
+    return 0;
+}
+
 void *bhhip_stream(void) { return g_stream; }
 
 int bhhip_occupancy(void *function, uint32_t block, int *blocks_per_cu) {

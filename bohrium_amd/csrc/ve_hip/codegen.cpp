@@ -673,13 +673,16 @@ void Writer::emit_instr(const bh_instruction &instr, Scope &sc, int ind) {
         const std::string c = const_ref(instr);
         ops.push_back("bh_random123(" + c + "_start, " + c + "_key, " + index(instr.operand[0]) + ")");
     } else if (instr.opcode == BH_GATHER) {
-        // out[...] = in[in.start + idx[...]], with the index checked against the base.
+        // out[...] = in[in.start + idx[...]], with the index checked against the base.  No branch: an
+        // index outside the base loads element 0 and marks bh_bad_, which raises the error flag when
+        // the function ends; a body without branches keeps the loads of several elements in flight.
         const bh_view &in = instr.operand[1];
         const int k = os_id(in);
         const std::string out = ref(sc, instr.operand[0]);
         body << "{ const int64_t g_ = vo" << k << " + (int64_t)(" << ref(sc, instr.operand[2]) << "); "
-             << "if (g_ < 0 || g_ >= nb" << bid.at(in.base) << ") { atomicOr(bh_err, BH_ERR_INDEX); } else { "
-             << out << " = a" << bid.at(in.base) << "[g_]; } }\n";
+             << "const bool ok_ = (uint64_t)g_ < (uint64_t)nb" << bid.at(in.base) << "; bh_bad_ |= !ok_; "
+             << out << " = a" << bid.at(in.base) << "[ok_ ? g_ : (int64_t)0]; }\n";
+        kp.uses_gather_flag = true;
         kp.uses_index_check = true;
         kp.uses_error_flag = true;
         return;
@@ -1349,26 +1352,37 @@ void Writer::emit_scan() {
             "__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
     body << "  }\n";
     body << "  __syncthreads();\n";
-    body << "  const int64_t tile_ = bh_tile_s;\n  const unsigned int fb_ = bh_ep_s << 2;\n";
+    body << "  const int64_t tile_ = (int64_t)__builtin_amdgcn_readfirstlane(bh_tile_s);\n"
+            "  const unsigned int fb_ = bh_ep_s << 2;\n";
     body << "  const int lane_ = threadIdx.x & 63, wave_ = threadIdx.x >> 6;\n";
     body << "  const int64_t tb_ = tile_ * " << tile_elems << ";\n";
     body << "  " << T << " v_[" << items << "];\n";
-    body << "  " << T << " carry_ = " << ident << ";\n";
     Scope ksc;
     declare_scope(unit, ksc, 1);
-    body << "  for (int j_ = 0; j_ < " << items << "; ++j_) {\n";
-    body << "    const int64_t i0 = tb_ + ((int64_t)wave_ * " << items << " + j_) * 64 + lane_;\n";
-    body << "    " << T << " x_ = " << ident << ";\n";
-    body << "    if (i0 < n" << lid.at(&l0) << ") {\n";
-    {
-        Scope sc;
-        sc.parent = &ksc;
-        declare_scope(l0, sc, 3);
-        emit_blocks(l0, sc, 3);
-        write_back(l0, sc, 3);
+    // The elements first (every load of the tile issued before the first wave scan waits for one):
+    // a full tile without bounds checks, unrolled; the last tile with them.
+    for (int full = 1; full >= 0; --full) {
+        if (full)
+            body << "  if (tb_ + " << tile_elems << " <= n" << lid.at(&l0) << ") {\n#pragma unroll\n";
+        else
+            body << "  } else {\n";
+        body << "  for (int j_ = 0; j_ < " << items << "; ++j_) {\n";
+        body << "    const int64_t i0 = tb_ + ((int64_t)wave_ * " << items << " + j_) * 64 + lane_;\n";
+        body << "    " << T << " x_ = " << ident << ";\n";
+        body << (full ? "    {\n" : "    if (i0 < n" + std::to_string(lid.at(&l0)) + ") {\n");
+        {
+            Scope sc;
+            sc.parent = &ksc;
+            declare_scope(l0, sc, 3);
+            emit_blocks(l0, sc, 3);
+            write_back(l0, sc, 3);
+        }
+        body << "    }\n    v_[j_] = x_;\n  }\n";
     }
-    body << "    }\n";
-    body << "    x_ = op_(carry_, bh_wave_inclusive_scan(x_, op_));\n";
+    body << "  }\n";
+    body << "  " << T << " carry_ = " << ident << ";\n";
+    body << "  for (int j_ = 0; j_ < " << items << "; ++j_) {\n";
+    body << "    const " << T << " x_ = op_(carry_, bh_wave_inclusive_scan(v_[j_], op_));\n";
     body << "    carry_ = bh_lane(x_, 63);\n";
     body << "    v_[j_] = x_;\n  }\n";
     body << "  if (lane_ == 63) bh_wtot_[wave_] = carry_;\n  __syncthreads();\n";
@@ -1467,9 +1481,14 @@ KernelPlan Writer::plan(Strategy s) {
     }
     std::ostringstream dev;
     const std::string main_src = s == Strategy::FULL_REDUCE ? main_body : body.str();
+    // Gathers mark an index outside its base in bh_bad_; the flag is raised once per thread at the end
+    // (every early return in a body comes before its first gather).
+    const std::string bad0 = kp.uses_gather_flag ? "  bool bh_bad_ = false;\n" : "";
+    const std::string bad1 = kp.uses_gather_flag ? "  if (bh_bad_) atomicOr(bh_err, BH_ERR_INDEX);\n" : "";
     dev << "__device__ __forceinline__ void bh_main_(" << plist << (plist.empty() ? "" : ",\n    ")
-        << "const unsigned bh_bid, const unsigned bh_nblk) {\n" << main_src << "}\n";
-    if (s == Strategy::FULL_REDUCE) dev << "\n__device__ __forceinline__ void bh_fin_(" << plist << ") {\n" << body.str() << "}\n";
+        << "const unsigned bh_bid, const unsigned bh_nblk) {\n" << bad0 << main_src << bad1 << "}\n";
+    if (s == Strategy::FULL_REDUCE)
+        dev << "\n__device__ __forceinline__ void bh_fin_(" << plist << ") {\n" << bad0 << body.str() << bad1 << "}\n";
     kp.device_code = dev.str();
     std::ostringstream src;
     // Occupancy hint for the main kernel (waves per SIMD the register allocator must allow).
@@ -1781,61 +1800,55 @@ std::vector<size_t> arg_offsets(const std::vector<ArgSpec> &args, size_t *total)
     return off;
 }
 
-std::string chain_source(const std::vector<const KernelPlan *> &stages, std::vector<size_t> &offsets,
-                         size_t *stride) {
+std::string persist_source(const std::vector<const KernelPlan *> &codes) {
     std::ostringstream s;
-    offsets.clear();
-    size_t at = 0;
-    for (size_t k = 0; k < stages.size(); ++k) {
-        const KernelPlan &p = *stages[k];
-        const std::string tag = "_s" + std::to_string(k) + "_";
+    std::ostringstream cases;
+    for (size_t c = 0; c < codes.size(); ++c) {
+        const KernelPlan &p = *codes[c];
+        const std::string tag = "_c" + std::to_string(c) + "_";
         std::string code = p.device_code;
-        for (const char *fn : {"bh_main_(", "bh_fin_("}) {
-            const std::string from = std::string("void ") + fn;
-            const std::string to = std::string("void ") + std::string(fn).substr(0, std::string(fn).size() - 2) + tag + "(";
+        for (const char *fn : {"bh_main_", "bh_fin_"}) {
+            const std::string from = std::string("void ") + fn + "(";
+            const std::string to = std::string("void ") + fn + tag.substr(1) + "(";
             for (size_t pos = code.find(from); pos != std::string::npos; pos = code.find(from, pos + to.size()))
                 code.replace(pos, from.size(), to);
         }
         s << code << "\n";
-        offsets.push_back(at);
-        size_t packed = 0;
-        arg_offsets(p.args, &packed);
-        at += (packed + 15) / 16 * 16 + 16;  // the stage's packed arguments, then its launch grid (uint32)
-    }
-    *stride = at;
-    // One workgroup per item (items ordered by the host, largest first); the stages run one after the
-    // other with a workgroup barrier between them: every stage with one workgroup (a full reduction
-    // leaves one partial, which its finalize folds; a scan takes its tiles in order).
-    s << "\nextern \"C\" __global__ void __launch_bounds__(256) bh_kernel_chain("
-      << "const unsigned char *__restrict__ bh_args, const unsigned *__restrict__ bh_order, const unsigned bh_stride) {\n"
-      << "  const unsigned char *I_ = bh_args + (size_t)bh_order[blockIdx.x] * bh_stride;\n";
-    for (size_t k = 0; k < stages.size(); ++k) {
-        const KernelPlan &p = *stages[k];
         const std::vector<size_t> off = arg_offsets(p.args, nullptr);
         std::string unpack;
         for (size_t i = 0; i < p.param_types.size(); ++i) {
             const std::string &t = p.param_types[i];
             const bool ptr = !t.empty() && t.back() == '*';
-            unpack += std::string(i ? ",\n      " : "") + "*(" + t + (ptr ? "const *" : " const *") + ")(A_ + " +
+            unpack += std::string(i ? ", " : "") + "*(" + t + (ptr ? "const *" : " const *") + ")(A_ + " +
                       std::to_string(off[i]) + ")";
         }
         const std::string sep = unpack.empty() ? "" : ", ";
-        const std::string tag = "_s" + std::to_string(k) + "_";
-        size_t packed = 0;
-        arg_offsets(p.args, &packed);
-        s << "  {\n    const unsigned char *A_ = I_ + " << offsets[k] << ";\n";
-        if (p.strategy == Strategy::SCAN) {
-            // The item's tiles in order: each call takes the next ticket, whose predecessors are done.
-            s << "    const unsigned nt_ = *(const unsigned *)(A_ + " << (packed + 15) / 16 * 16 << ");\n"
-              << "    for (unsigned t_ = 0; t_ < nt_; ++t_) {\n      bh_main" << tag << "(" << unpack << sep
-              << "t_, nt_);\n      __syncthreads();\n    }\n";
-        } else {
-            s << "    bh_main" << tag << "(" << unpack << sep << "0u, 1u);\n    __syncthreads();\n";
-            if (!p.fin_name.empty()) s << "    bh_fin" << tag << "(" << unpack << ");\n    __syncthreads();\n";
-        }
-        s << "  }\n";
+        cases << "        case " << 2 * c << "u: bh_main" << tag << "(" << unpack << sep << "b_, nb_); break;\n";
+        if (!p.fin_name.empty()) cases << "        case " << 2 * c + 1 << "u: bh_fin" << tag << "(" << unpack << "); break;\n";
     }
-    s << "}\n";
+    // Workgroup w of the cooperative grid runs tasks w, w + grid, ... of each level, the levels in
+    // order with a grid barrier between them.  A task is (case, workgroup number within its launch,
+    // that launch's workgroup count, arguments offset / 16); case 2c runs code c's main body, 2c + 1
+    // its finalize; any other case (padding) nothing.  All control flow is wave-uniform.
+    s << "\nextern \"C\" __global__ void __launch_bounds__(256) bh_kernel_persist("
+      << "const unsigned char *__restrict__ bh_args, const uint4 *__restrict__ bh_tasks, "
+         "const unsigned *__restrict__ bh_levels, const unsigned bh_nlevels, unsigned *__restrict__ bh_bar, "
+         "unsigned *__restrict__ bh_gerr) {\n"
+      << "  for (unsigned L_ = 0; L_ < bh_nlevels; ++L_) {\n"
+      << "    const unsigned t1_ = __builtin_amdgcn_readfirstlane(bh_levels[L_ + 1u]);\n"
+      << "    for (unsigned t_ = __builtin_amdgcn_readfirstlane(bh_levels[L_]) + blockIdx.x; t_ < t1_; t_ += gridDim.x) {\n"
+      << "      const uint4 k_ = bh_tasks[t_];\n"
+      << "      const unsigned c_ = __builtin_amdgcn_readfirstlane(k_.x);\n"
+      << "      const unsigned b_ = __builtin_amdgcn_readfirstlane(k_.y);\n"
+      << "      const unsigned nb_ = __builtin_amdgcn_readfirstlane(k_.z);\n"
+      << "      const unsigned char *A_ = bh_args + (size_t)__builtin_amdgcn_readfirstlane(k_.w) * 16u;\n"
+      << "      switch (c_) {\n" << cases.str() << "        default: break;\n      }\n"
+      << "      __syncthreads();\n"
+      << "    }\n"
+      << "    if (L_ + 1u < bh_nlevels) bh_grid_sync(bh_bar, (L_ + 1u) * gridDim.x, bh_gerr);\n"
+      << "  }\n"
+      << "  bh_grid_exit(bh_bar);\n"
+      << "}\n";
     return s.str();
 }
 
@@ -1851,6 +1864,21 @@ std::string batch_source(const KernelPlan &p) {
     const std::string sep = unpack.empty() ? "" : ", ";
     std::ostringstream s;
     s << p.device_code;
+    if (p.strategy == Strategy::SCAN) {
+        // A scan tile takes its tile number from its item's ticket counter, so any workgroup may run
+        // any item: bh_first holds the item of each workgroup (the order is the host's; all ones: no
+        // item, the workgroup ends at once), then from position bh_items (the workgroup count) the tile
+        // count of each item.
+        s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ")" << p.attributes
+          << " bh_kernel_batch("
+          << "const unsigned char *__restrict__ bh_args, const unsigned *__restrict__ bh_first, const unsigned bh_items, "
+             "const unsigned bh_stride) {\n"
+          << "  const unsigned it_ = bh_first[blockIdx.x];\n"
+          << "  if (it_ == 0xffffffffu) return;\n"
+          << "  const unsigned char *A_ = bh_args + (size_t)it_ * bh_stride;\n"
+          << "  bh_main_(" << unpack << sep << "blockIdx.x, bh_first[bh_items + it_]);\n}\n";
+        return s.str();
+    }
     // Workgroup b of the batched grid runs workgroup b - first[i] of item i, where
     // first[i] <= b < first[i + 1]; each item's packed arguments sit at args + i * stride.
     s << "\nextern \"C\" __global__ void __launch_bounds__(" << p.block << ")" << p.attributes << " bh_kernel_batch("

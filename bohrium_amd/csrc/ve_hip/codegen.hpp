@@ -76,6 +76,7 @@ struct KernelPlan {
     int64_t scratch_bytes = 0;
     bool uses_index_check = false;  // gather/scatter: the flush checks the error flag when it ends
     bool uses_error_flag = false;   // takes bh_err (index checks, bounded look-back waits)
+    bool uses_gather_flag = false;  // a gather marks bad indices in bh_bad_ (raised when the body ends)
     uint64_t work_items = 0;  // elements of the iteration space (for statistics / cost model features)
     /* Filled when the plan is cached: ordinals of params/written bases and the compiled function. */
     std::vector<int> param_ordinals, written_ordinals;
@@ -134,12 +135,10 @@ std::vector<size_t> arg_offsets(const std::vector<ArgSpec> &args, size_t *total)
  * workgroup per item, for FULL_REDUCE). */
 std::string batch_source(const KernelPlan &plan);
 
-/* Source of the chained form of a sequence of plans (stages): bh_kernel_chain runs one item per
- * workgroup of 256 threads, all stages of the item in order with a barrier between them, each stage
- * as one workgroup of its skeleton (MAP, FULL_REDUCE with its finalize, SCAN tile by tile).  An item's
- * arguments are the stages' packed arguments at `offsets` (16-byte aligned), each followed by 16
- * bytes holding the stage's launch grid (uint32); `stride` is the item's total. */
-std::string chain_source(const std::vector<const KernelPlan *> &stages, std::vector<size_t> &offsets, size_t *stride);
+/* Source of a persistent launch running many plans of a flush (codes: one plan per distinct kernel
+ * code): bh_kernel_persist, launched cooperatively with 256-thread workgroups, runs a table of tasks
+ * level by level with a grid barrier between levels (kernel_prelude bh_grid_sync). */
+std::string persist_source(const std::vector<const KernelPlan *> &codes);
 
 /* HIP spelling of an element type inside generated kernels. */
 const char *hip_type(bh_type t);

@@ -162,6 +162,11 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     track_host_writes = cfg.defaultGet<bool>("host_write_tracking", true);
     graph_repeats = cfg.defaultGet<bool>("graph_repeats", true);
     batch_min = cfg.defaultGet<int>("batch_min", 8);
+    batch_scan_group = std::max(1, cfg.defaultGet<int>("batch_scan_group", 8));
+    batch_xcds = std::max(1, cfg.defaultGet<int>("batch_xcds", 1));
+    persistent = cfg.defaultGet<bool>("batch_persistent", false);
+    persist_items = std::max(0, cfg.defaultGet<int>("persist_items", 16384));
+    persist_grid = std::max(0, cfg.defaultGet<int>("persist_grid", 0));
     profile = cfg.defaultGet<bool>("prof", false);
     cache_dir = cfg.defaultGet<std::string>("cache_dir", "");
     if (cache_dir.empty() || cache_dir == "NONE") cache_dir = default_cache_dir();
@@ -491,7 +496,8 @@ void EngineHip::compilePlan(KernelPlan &plan) {
 }
 
 size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
-                           const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap) {
+                           const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap,
+                           uint32_t grid_arg) {
     size_t n = 0;
     auto put = [&](const void *p, int size, int align) {
         while (n % static_cast<size_t>(align)) {
@@ -522,9 +528,12 @@ size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> 
                 // (compile-only dumps stand a base's ordinal + 1, shifted, in for its address)
                 ptr = dry_run ? reinterpret_cast<void *>(static_cast<uintptr_t>(a.base_ordinal + 1) << 32)
                               : bufs.at(ord2base[a.base_ordinal]).ptr;
+                // A base without memory (no elements) stands on zeroed device memory: a gather reads
+                // element 0 of its source whatever the index, and raises the index error instead.
+                if (ptr == nullptr && !dry_run) ptr = counters + 16;
                 break;
             case ArgSpec::GRID_U32: {
-                const uint32_t g = static_cast<uint32_t>(plan.grid);
+                const uint32_t g = grid_arg != 0 ? grid_arg : static_cast<uint32_t>(plan.grid);
                 put(&g, 4, 4);
                 continue;
             }
@@ -658,6 +667,12 @@ void EngineHip::checkErrors() {
         const unsigned int zero = 0;
         check(bhhip_memcpy_h2d(counters + 1, &zero, sizeof(zero)), "clearing the error flag");
         if (flag & 1u) throw std::out_of_range("hip engine: gather/scatter index out of bounds");
+        if (flag & 4u) {
+            // the barrier counters of that launch may be left mid-count: no more persistent launches
+            persistent = false;
+            throw std::runtime_error("hip engine: a workgroup of a persistent launch gave up waiting at a grid "
+                                     "barrier (its results are wrong; persistent launches are now off)");
+        }
         throw std::runtime_error("hip engine: a scan tile gave up waiting for the prefix of an earlier tile "
                                  "(its accumulate results are wrong)");
     }
@@ -1179,6 +1194,7 @@ std::string EngineHip::statistics() const {
     ss << "graph_replays: " << graph_replays << "\ngraphs_cached: " << graph_cache.size() << "\n";
     ss << "graph_declined_changed: " << graph_declined_changed << "\ngraph_declined_busy: " << graph_declined_busy << "\n";
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";
+    ss << "persist_launches: " << persist_launches << "\npersist_fallbacks: " << persist_fallbacks << "\n";
     ss << "twin_exchanges: " << twin_exchanges << "\ntwin_full_copies: " << twin_full_copies << "\n";
     ss << "autotune_model_picks: " << tuner.model_picks << "\n";
     ss << "autotune_trials: " << tuner.trials << "\nautotune_decisions: " << tuner.decisions
@@ -1451,6 +1467,24 @@ public:
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
+        } else if (msg.rfind("batch_scan_group:", 0) == 0) {
+            // Batched scans: items whose tiles are interleaved (1: each item's tiles in a row).
+            engine.batch_scan_group = std::max(1, std::stoi(msg.substr(17)));
+            engine.dropPlans();
+        } else if (msg == "batch_persistent: on" || msg == "batch_persistent: off") {
+            // Batched schedules as one persistent launch, or one launch per level and group.
+            engine.persistent = msg == "batch_persistent: on";
+        } else if (msg.rfind("persist_items:", 0) == 0 || msg.rfind("persist_grid:", 0) == 0) {
+            // Elements per task of grid-stride plans, or the cap on the persistent grid (0: none);
+            // drops cached plans (and with them their persistent plans).
+            const bool items = msg.rfind("persist_items:", 0) == 0;
+            const int v = std::max(0, std::stoi(msg.substr(items ? 14 : 13)));
+            (items ? engine.persist_items : engine.persist_grid) = v;
+            engine.dropPlans();
+        } else if (msg.rfind("batch_xcds:", 0) == 0) {
+            // Batched scans: XCDs the workgroups are dealt to round robin (1: no XCD placement).
+            engine.batch_xcds = std::max(1, std::stoi(msg.substr(11)));
+            engine.dropPlans();
         } else if (msg == "plans: drop") {
             engine.dropPlans();
         } else if (msg == "autotune: on" || msg == "autotune: off") {

@@ -132,6 +132,11 @@ public:
     uint64_t kernel_seq = 0;  // id of the jitk kernel whose plans are being launched
     /* Batched launches: groups of at least `batch_min` launches of one kernel code (0 disables). */
     int batch_min = 8;
+    /* Batched scans: items whose tiles are interleaved with each other (1: each item's tiles in a row). */
+    int batch_scan_group = 8;
+    /* Batched scans: XCDs the workgroups go to round robin (all tiles of an item on one; 1, the
+     * default: no placement, which measured the same on the batched-arrays workload). */
+    int batch_xcds = 1;
     uint64_t batch_launches = 0, batched_items = 0;
     void setBatchMin(int n) {
         batch_min = n;
@@ -210,7 +215,7 @@ private:
     uint64_t reduce_scratch_bytes = 0;
     unsigned char *scan_scratch = nullptr;
     uint64_t scan_scratch_bytes = 0;
-    unsigned int *counters = nullptr;  // [0]: reduce ticket, [1]: error flag
+    unsigned int *counters = nullptr;  // [0]: reduce ticket, [1]: error flag, [16..63]: zeros (stand-in base)
     EngineStats stats;
     /* Autotuning of one kernel in progress: its plans under each distinct candidate knob set,
      * device times measured so far, and launches whose events are not read yet. */
@@ -259,9 +264,36 @@ private:
         std::vector<size_t> groups;
         std::vector<uint32_t> free_kernels;  // kernels whose frees are released after this level
     };
+    /* The whole schedule as one cooperative launch of bh_kernel_persist (persist_source): the
+     * levels' launches become tasks of a table, run level by level with a grid barrier between
+     * levels (and between a level's full-reduction main bodies and their finalizes). */
+    struct PersistPlan {
+        std::vector<PlanRef> plans;         // every plan of the schedule (argument blocks in this order)
+        std::vector<const KernelPlan *> codes;  // one plan per distinct kernel code
+        std::vector<uint64_t> arg_off;      // per plan: byte offset of its packed arguments (16-aligned)
+        uint64_t args_bytes = 0;
+        std::vector<uint64_t> scratch_off;  // per plan: byte offset of its scratch region
+        std::vector<uint32_t> nblk;         // per plan: its workgroups (tasks) in the persistent launch
+        uint64_t scratch_bytes = 0;
+        std::vector<uint32_t> tasks;        // 4 words per task: case, workgroup, workgroups, args / 16
+        std::vector<uint32_t> levels;       // first task of each level, then the end
+        uint32_t max_level_tasks = 0;
+        void *fn = nullptr;
+        uint32_t grid = 0;
+        unsigned char *dev = nullptr;  // tasks | levels | barrier counters (16 bytes) | arguments
+        uint64_t dev_bytes = 0, args_at = 0;
+        bool tables_uploaded = false;
+        uint8_t *host = nullptr;  // page-locked staging
+        uint64_t host_bytes = 0;
+        void *uploaded = nullptr;
+        unsigned char *scratch = nullptr;  // zeroed once (scan look-back state persists across launches)
+        ~PersistPlan();
+    };
     struct Schedule {
         std::vector<Level> levels;
         std::vector<std::unique_ptr<BatchGroup> > groups;
+        std::unique_ptr<PersistPlan> persist;
+        bool persist_checked = false;  // persist built, or found not to apply
     };
     struct CachedFlush {
         std::vector<CachedKernel> kernels;
@@ -276,8 +308,27 @@ private:
                       const char *dir);
     void launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh_base *> &ord2base,
                      const std::vector<bh_instruction *> &instrs);
+    /* Build the schedule's persistent plan (null when a plan does not fit the persistent kernel). */
+    std::unique_ptr<PersistPlan> buildPersist(CachedFlush &cf);
+    /* False when the cooperative launch could not be made (the caller runs the levels). */
+    bool launchPersist(PersistPlan &pp, CachedFlush &cf, const std::vector<bh_base *> &ord2base,
+                       const std::vector<bh_instruction *> &instrs);
+public:
+    /* Batched schedules run as one persistent launch (false, the default: one launch per level and
+     * group, which measures faster on MI355X: the grid barriers' cache write-back and invalidation and
+     * the task loop cost more than the launches they replace). */
+    bool persistent = false;
+    /* Grid-stride plans (map, full and row reductions) take one task per `persist_items` elements (at
+     * most their own workgroup count): fewer, longer tasks than their standalone grid.  0: the plan's
+     * own grid. */
+    int persist_items = 16384;
+    int persist_grid = 0;  // >0: at most this many workgroups in the persistent launch
+    uint64_t persist_launches = 0, persist_fallbacks = 0;
+private:
+    /* `grid_arg`: the value packed for a GRID_U32 argument (0: the plan's grid). */
     size_t packArgs(const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
-                    const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap);
+                    const std::vector<bh_instruction *> &instrs, void *scratch, uint8_t *out, size_t cap,
+                    uint32_t grid_arg = 0);
     void compilePlan(KernelPlan &plan);
     void dumpLaunch(const char *dir, const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
                     const std::vector<bh_instruction *> &instrs);

@@ -73,6 +73,9 @@ int bhhip_launch(void *function, uint64_t grid, uint32_t block, const void *args
  * kernels, whose launch shape the caller chooses (reference ve/opencl/engine_opencl.cpp:632-662). */
 int bhhip_launch_3d(void *function, const uint32_t grid[3], const uint32_t block[3], const void *args,
                     size_t args_bytes);
+/* Launch `function` cooperatively (every workgroup of the grid resident at once, or an error) with
+ * `params` pointing at each argument in turn; for kernels whose workgroups wait for each other. */
+int bhhip_launch_cooperative(void *function, uint32_t grid, uint32_t block, void **params);
 int bhhip_synchronize(void);
 /* Workgroups of `block` threads of `function` that can be resident on one compute unit at once. */
 int bhhip_occupancy(void *function, uint32_t block, int *blocks_per_cu);

@@ -163,3 +163,124 @@ print(bh.message("plan log"))
     assert r.returncode == 0, r.stdout[-3000:]
     grids = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("reduce ")]
     assert grids and 5 in grids and 1 not in grids, r.stdout
+
+
+BATCH_DRIVER = r"""
+import sys
+import numpy as np
+sys.path.insert(0, %(root)r)
+import bohrium_amd as bh
+from bohrium_amd import workloads, _bh_api
+_bh_api.set_flush_threshold(10 ** 7)
+bh.message("batch_persistent: on")
+bh.message("batch_xcds: 8")
+sizes = workloads.batched_sizes(40, 256, 40000, seed=7)
+xs, idxs = workloads.batched_inputs(sizes, seed=7)
+bh.flush()
+sums, cums, gathered = workloads.batched_step(xs, idxs)
+bh.flush()
+print("SIZES=" + ",".join(str(int(n)) for n in sizes))
+"""
+
+
+def test_batched_scan_tiles_are_placed_per_xcd(tmp_path):
+    """A batched scan launch (40 uneven arrays, 1-20 tiles of 2048 elements each) gives every
+    workgroup an item or none: each item gets exactly as many workgroups as it has tiles, all with the
+    same workgroup number mod 8 (the XCD the workgroup goes to, whose L2 then holds the item's gather
+    source), the XCDs' tile counts differ by at most one item's, and the workgroups without an item
+    only pad the XCDs to the same count."""
+    env = dict(os.environ, BH_HIP_DRY_RUN="1", BH_HIP_DUMP_DIR=str(tmp_path))
+    r = subprocess.run([sys.executable, "-c", BATCH_DRIVER % {"root": ROOT}], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:]
+    sizes = [int(v) for v in [l for l in r.stdout.splitlines() if l.startswith("SIZES=")][0][6:].split(",")]
+    scans = []
+    for f in sorted(tmp_path.glob("batch_*.txt")):
+        fields = {}
+        for line in f.read_text().splitlines():
+            if line == "code":
+                break
+            k, _, v = line.partition(" ")
+            fields[k] = v
+        if fields["strategy"] == "scan":
+            scans.append(fields)
+    assert len(scans) == 1
+    s = scans[0]
+    items, blocks = int(s["items"]), int(s["blocks"])
+    assert int(s["items_arg"]) == blocks and blocks % 8 == 0
+    table = [int(v) for v in s["table"].split()]
+    assert items == len(sizes) and len(table) == blocks + items
+    counts = table[blocks:]
+    assert sorted(counts) == sorted((n + 2047) // 2048 for n in sizes)
+    seen = [0] * items
+    xcd = {}
+    load = [0] * 8
+    for b in range(blocks):
+        it = table[b]
+        if it == 0xFFFFFFFF:
+            continue
+        seen[it] += 1
+        assert xcd.setdefault(it, b % 8) == b % 8
+        load[b % 8] += 1
+    assert seen == counts
+    assert max(load) - min(load) <= max(counts)
+    assert blocks == 8 * max(load)
+
+
+def test_persistent_plan_covers_every_launch_in_dependency_order(tmp_path):
+    """The batched schedule as one persistent launch (40 uneven arrays: map+sum, then scan+gather of
+    the map's output): every workgroup of every plan is one task (grid-stride plans with fewer, longer
+    workgroups than their own grid, scans with one per tile), a full reduction's finalize comes in
+    a later level than all its main workgroups, the scans (which gather the map's output) in a later
+    level than every map workgroup, and each scan item's tiles share one XCD (position in the level
+    mod 8)."""
+    env = dict(os.environ, BH_HIP_DRY_RUN="1", BH_HIP_DUMP_DIR=str(tmp_path))
+    r = subprocess.run([sys.executable, "-c", BATCH_DRIVER % {"root": ROOT}], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:]
+    plans = []
+    for f in sorted(tmp_path.glob("persist_*.txt")):
+        fields = {}
+        for line in f.read_text().splitlines():
+            if line == "code":
+                break
+            k, _, v = line.partition(" ")
+            fields[k] = v
+        if "reduce" in fields.get("code_strategies", ""):
+            plans.append(fields)
+    assert len(plans) == 1
+    p = plans[0]
+    strategies = p["code_strategies"].split()
+    grids = [int(v) for v in p["grids"].split()]
+    nblk = [int(v) for v in p["nblk"].split()]
+    arg_off = [int(v) for v in p["arg_off"].split()]
+    levels = [int(v) for v in p["levels"].split()]
+    words = [int(v) for v in p["tasks"].split()]
+    tasks = [tuple(words[i:i + 4]) for i in range(0, len(words), 4)]
+    assert levels[0] == 0 and levels[-1] == len(tasks) and levels == sorted(levels)
+    plan_of = {off // 16: i for i, off in enumerate(arg_off)}
+    level_of = lambda t: max(L for L in range(len(levels) - 1) if levels[L] <= t)
+    mains, fins = {}, {}
+    for t, (case, b, nb, off16) in enumerate(tasks):
+        if case == 0xFFFFFFFF:
+            continue
+        i = plan_of[off16]
+        assert nb == (1 if case % 2 else nblk[i]) and 1 <= nblk[i] <= grids[i]
+        (fins if case % 2 else mains).setdefault(i, []).append((t, b, strategies[case // 2]))
+    assert sorted(mains) == list(range(len(grids)))
+    for i, ts in mains.items():
+        kind = ts[0][2]
+        assert len(ts) == nblk[i]
+        if kind != "scan":
+            assert sorted(b for _, b, _ in ts) == list(range(nblk[i]))
+        else:
+            assert nblk[i] == grids[i]
+            assert len({(t - levels[level_of(t)]) % 8 for t, _, _ in ts}) == 1
+    reduce_plans = [i for i, ts in mains.items() if ts[0][2] == "reduce"]
+    scan_plans = [i for i, ts in mains.items() if ts[0][2] == "scan"]
+    assert len(reduce_plans) == len(scan_plans) == 40 and sorted(fins) == sorted(reduce_plans)
+    last_map_level = max(level_of(t) for i in reduce_plans for t, _, _ in mains[i])
+    for i in reduce_plans:
+        assert level_of(fins[i][0][0]) > last_map_level
+    for i in scan_plans:
+        assert min(level_of(t) for t, _, _ in mains[i]) > last_map_level

@@ -387,29 +387,42 @@ def _batched_prog(count, seed=7):
     return prog, count
 
 
+BATCH_MODES = {"persistent": ("batch_persistent: on", "persist_launches:"),
+               "levels": ("batch_persistent: off", "batch_launches:")}
+
+
+def _stat(stats, name):
+    return int([l for l in stats.splitlines() if l.startswith(name)][0].split()[1])
+
+
+@pytest.mark.parametrize("mode", sorted(BATCH_MODES))
 @pytest.mark.parametrize("count", [16, 200])
-def test_batched_uneven_blocks_parity(count):
-    """Many uneven arrays in one flush: the map+sum, scan+gather launches of every array run
-    as batched launches; results match the oracle (sums fp32, cumsums and gathers exact) and
-    the unbatched engine bit for bit."""
+def test_batched_uneven_blocks_parity(count, mode):
+    """Many uneven arrays in one flush: the map+sum, scan+gather launches of every array run as
+    one persistent launch (or as batched launches level by level); results match the oracle (sums
+    fp32, cumsums and gathers exact) and the unbatched engine bit for bit (the persistent launch's
+    sums within the summation-order bound: it splits a sum into fewer, longer workgroups)."""
     from bohrium_amd import _bh_api as api
     prog, n = _batched_prog(count)
+    msg, counter = BATCH_MODES[mode]
     api.set_flush_threshold(10 ** 7)
     try:
         bh.message("batch_min: 0")
         plain = [a.asnumpy() for a in prog()]
         bh.message("batch_min: 8")
-        stats0 = bh.message("statistic")
-        before = int([l for l in stats0.splitlines() if l.startswith("batch_launches:")][0].split()[1])
+        bh.message(msg)
+        before = _stat(bh.message("statistic"), counter)
         got, want = run_both(prog)
         stats1 = bh.message("statistic")
-        after = int([l for l in stats1.splitlines() if l.startswith("batch_launches:")][0].split()[1])
+        after = _stat(stats1, counter)
     finally:
         api.set_flush_threshold(1000)
         bh.message("batch_min: 8")
-    assert after > before, "no batched launch:\n" + stats1
-    for g, p in zip(got, plain):
-        np.testing.assert_array_equal(g, p)
+        bh.message("batch_persistent: off")
+    assert after > before, "no %s launch:\n" % mode + stats1
+    for i, (g, p) in enumerate(zip(got, plain)):
+        if mode == "levels" or i >= n:
+            np.testing.assert_array_equal(g, p)
     for i in range(n):
         # fp32 sums in different orders: both within the bound n * 2^-24 * sum|y| of the exact sum
         y = np.tanh(want[3 * n + i].astype(np.float64) * 0.5 + 0.1)
@@ -456,20 +469,23 @@ def test_row_reductions_one_wave_per_row(shape, dtype):
         assert_parity(g, p, g.dtype, rtol=rtol, atol=0.0 if g.dtype.kind != "f" else 1e-6)
 
 
-def test_batched_full_configuration():
+@pytest.mark.parametrize("mode", sorted(BATCH_MODES))
+def test_batched_full_configuration(mode):
     """The batched-arrays configuration at its full size: 4096 float32 arrays of
     floor(exp(U[ln 256, ln 262144])) elements (seed 42, 153,874,193 elements), x ~ N(0, 1) and
-    int64 indices from philox, y = tanh(0.5 x + 0.1), sum(y), int64 cumsum(x > 0), y[idx], run as
-    batched launches.  The oracle would need minutes on the host at this size, so each result is
+    int64 indices from philox, y = tanh(0.5 x + 0.1), sum(y), int64 cumsum(x > 0), y[idx], run as one
+    persistent launch (or as batched launches level by level).  The oracle would need minutes on the host at this size, so each result is
     checked against what the fetched inputs fix: the cumsums equal numpy's exactly, every gathered
     element is y at its index (tanh within 1e-5 relative), and every sum lies within
     n * 2^-24 * sum|y| (summation order) + 1e-5 * sum|y| (tanh) of the float64 sum of y."""
     from bohrium_amd import _bh_api as api
     sizes = workloads.batched_sizes(4096)
     assert int(sizes.sum()) == 153874193 and sizes.min() >= 256 and sizes.max() <= 262144
+    msg, counter = BATCH_MODES[mode]
     api.set_flush_threshold(10 ** 7)
     try:
         bh.message("batch_min: 8")
+        bh.message(msg)
         xs, idxs = workloads.batched_inputs(sizes)
         bh.flush()
         stats0 = bh.message("statistic")
@@ -482,8 +498,8 @@ def test_batched_full_configuration():
         host_i = bh.asnumpy_many(idxs)
     finally:
         api.set_flush_threshold(1000)
-    batches = lambda st: int([l for l in st.splitlines() if l.startswith("batch_launches:")][0].split()[1])
-    assert batches(stats1) > batches(stats0), "no batched launch"
+        bh.message("batch_persistent: off")
+    assert _stat(stats1, counter) > _stat(stats0, counter), "no %s launch" % mode
     for k in range(len(sizes)):
         x, idx = host_x[k], host_i[k]
         n = int(sizes[k])
