@@ -225,7 +225,8 @@ def jacobi_secondary(bh, workloads):
 
 
 def batched_secondary(bh, workloads, api):
-    """4096 uneven float32 arrays (256..262144 elements) in one flush: batched vs one launch per block."""
+    """4096 uneven float32 arrays (256..262144 elements) in one flush: batched launches level by level
+    (the default), the whole schedule as one persistent launch, and one launch per block."""
     sizes = workloads.batched_sizes(4096)
     elems = int(sizes.sum())
     api.set_flush_threshold(10 ** 7)
@@ -239,8 +240,9 @@ def batched_secondary(bh, workloads, api):
         sums, cums, gathered = workloads.batched_step(xs, idxs)
         bh.asnumpy_many([sums[-1], cums[-1][:1], gathered[-1][:1]])
 
-    for label, bm in (("batched", 8), ("one_launch_per_block", 0)):
+    for label, bm, persistent in (("batched", 8, "off"), ("persistent", 8, "on"), ("one_launch_per_block", 0, "off")):
         bh.message("batch_min: %d" % bm)
+        bh.message("batch_persistent: " + persistent)
         run()
         t0 = time.perf_counter()
         run()
@@ -254,6 +256,7 @@ def batched_secondary(bh, workloads, api):
         out[label] = {"launches": len(ls), "device_ms": round(dev, 3), "wall_ms": round(dt * 1e3, 1),
                       "device_GBs": round(32.0 * elems / (dev * 1e-3) / 1e9, 1) if dev > 0 else None}
     bh.message("batch_min: 8")
+    bh.message("batch_persistent: off")
     api.set_flush_threshold(1000)
     return out
 

@@ -175,7 +175,14 @@ def _rec(*record):
 # ---------------------------------------------------------------- API
 
 def new(dtype, nelem):
-    dtype = np.dtype(dtype)
+    if _fast is not None and _trace is None and _initialised[0] and not _record_only[0] and type(nelem) is int:
+        # the common case: one call into the extension
+        try:
+            return _fast.new(BH_TYPES[dtype if isinstance(dtype, np.dtype) else np.dtype(dtype)], nelem)
+        except RuntimeError as e:
+            raise BohriumError(str(e))
+    if not isinstance(dtype, np.dtype):
+        dtype = np.dtype(dtype)
     if _record_only[0]:
         _fake_handle[0] += 1
         h = ("fake", _fake_handle[0])
@@ -216,9 +223,20 @@ def op(opcode, operands, const=None, slides=None):
     (a numpy scalar of the constant's type).  `slides` optionally gives, per
     operand, a tuple of slide dims (rank, offset_change, shape_change, stride,
     shape, step_delay) that move the view between repeats."""
+    if _fast is not None and _trace is None and not slides and _initialised[0] and not _record_only[0]:
+        # the common case: one call into the extension
+        try:
+            if const is None:
+                _fast.op(opcode, operands, -1, None)
+            else:
+                c = const if isinstance(const, np.generic) else np.asarray(const)
+                _fast.op(opcode, operands, BH_TYPES[c.dtype], c.tobytes())
+        except RuntimeError as e:
+            raise BohriumError(str(e))
+        return
     ctype, cbytes = -1, None
     if const is not None:
-        c = np.asarray(const)
+        c = const if isinstance(const, np.generic) else np.asarray(const)
         ctype = BH_TYPES[c.dtype]
         cbytes = c.tobytes()
     if not _record_only[0]:
@@ -265,6 +283,14 @@ def random123(handle, start, shape, stride, r_start, r_key):
 
 
 def free(handle):
+    if _fast is not None and _trace is None and _initialised[0] and not _fake_handles:
+        try:
+            _fast.free(handle)
+        except RuntimeError as e:
+            raise BohriumError(str(e))
+        if _base_ids:
+            _retire(handle)
+        return
     if handle in _fake_handles:
         _fake_handles.discard(handle)
         if _trace is not None:

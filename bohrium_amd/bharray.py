@@ -14,13 +14,16 @@ from . import _bh_api as api
 from . import opcodes as ops
 
 
+_DTYPE = type(np.dtype(np.float64))
+
+
 class BhBase:
     """Owner of one bridge base array; queues BH_FREE when collected."""
 
     __slots__ = ("handle", "dtype", "nelem", "__weakref__")
 
     def __init__(self, dtype, nelem):
-        self.dtype = np.dtype(dtype)
+        self.dtype = dtype if type(dtype) is _DTYPE else np.dtype(dtype)
         self.nelem = int(nelem)
         self.handle = api.new(self.dtype, max(1, self.nelem))
 
@@ -32,12 +35,20 @@ class BhBase:
             pass
 
 
+_STRIDES = {}
+
+
 def _contiguous_strides(shape):
-    st, s = [], 1
-    for d in reversed(shape):
-        st.append(s)
-        s *= d
-    return tuple(reversed(st))
+    st = _STRIDES.get(shape)
+    if st is None:
+        acc, s = [], 1
+        for d in reversed(shape):
+            acc.append(s)
+            s *= d
+        st = tuple(reversed(acc))
+        if len(_STRIDES) < 65536:
+            _STRIDES[shape] = st
+    return st
 
 
 class BhArray:
@@ -46,16 +57,27 @@ class BhArray:
     __array_priority__ = 100
 
     def __init__(self, shape, dtype=np.float64, base=None, offset=0, strides=None, slides=()):
-        if isinstance(shape, numbers.Integral):
+        if type(shape) is tuple:
+            for s in shape:
+                if type(s) is not int:
+                    shape = tuple(int(s) for s in shape)
+                    break
+        elif isinstance(shape, numbers.Integral):
             shape = (int(shape),)
-        self.shape = tuple(int(s) for s in shape)
-        self.dtype = np.dtype(dtype) if base is None else base.dtype
-        n = 1
-        for s in self.shape:
-            n *= s
-        self.base = base if base is not None else BhBase(self.dtype, n)
-        self.offset = int(offset)
-        self.strides = tuple(strides) if strides is not None else _contiguous_strides(self.shape)
+        else:
+            shape = tuple(int(s) for s in shape)
+        self.shape = shape
+        if base is None:
+            self.dtype = dtype if type(dtype) is _DTYPE else np.dtype(dtype)
+            n = 1
+            for s in shape:
+                n *= s
+            self.base = BhBase(self.dtype, n)
+        else:
+            self.dtype = base.dtype
+            self.base = base
+        self.offset = offset if type(offset) is int else int(offset)
+        self.strides = _contiguous_strides(shape) if strides is None else tuple(strides)
         # Slide dims (rank, offset_change, shape_change, stride, shape, step_delay) applied between
         # the repeats of do_while (views indexed by an Iterator).
         self.slides = tuple(slides)
@@ -83,7 +105,7 @@ class BhArray:
 
     def view_desc(self):
         """(handle, start, shape, stride) of this view for the bridge; 0-d views are 1-element 1-d views."""
-        if self.ndim == 0:
+        if not self.shape:
             return (self.base.handle, self.offset, (1,), (1,))
         return (self.base.handle, self.offset, self.shape, self.strides)
 
@@ -353,8 +375,10 @@ def _as_operand(x):
 
 
 def _slides_of(arrays):
-    s = [a.slides if isinstance(a, BhArray) else () for a in arrays]
-    return s if any(s) else None
+    for a in arrays:
+        if isinstance(a, BhArray) and a.slides:
+            return [a.slides if isinstance(a, BhArray) else () for a in arrays]
+    return None
 
 
 def _same_memory(a, b):
@@ -370,8 +394,19 @@ def ops_identity(src, dst):
         api.op(ops.IDENTITY, [dst.view_desc(), None], const=_const_scalar(src, dst.dtype), slides=_slides_of([dst]))
 
 
+_CONSTS = {}
+
+
 def _const_scalar(value, dtype):
     """A constant in the instruction's type, converted the way the reference's bridges do (C cast)."""
+    if type(value) in (int, float, bool):
+        key = (type(value), value, dtype)
+        c = _CONSTS.get(key)
+        if c is None:
+            c = np.asarray(value).astype(np.dtype(dtype))[()]
+            if len(_CONSTS) < 65536:
+                _CONSTS[key] = c
+        return c
     return np.asarray(value).astype(np.dtype(dtype))[()]
 
 
@@ -557,7 +592,12 @@ class Ufunc:
             raise TypeError("%s takes %d inputs" % (self.name, self.nin))
         in_types, out_type = self._loop_dtypes(args)
         shapes = [a.shape for a in args if isinstance(a, BhArray)]
-        shape = np.broadcast_shapes(*shapes) if shapes else ()
+        if not shapes:
+            shape = ()
+        elif len(shapes) == 1 or all(sh == shapes[0] for sh in shapes):
+            shape = shapes[0]
+        else:
+            shape = np.broadcast_shapes(*shapes)
         if out is not None:
             shape = out.shape
         operands = []
@@ -566,7 +606,7 @@ class Ufunc:
             if isinstance(a, BhArray):
                 if a.dtype != t:
                     a = a.astype(t)
-                operands.append(_broadcast_view(a, shape))
+                operands.append(a if a.shape == shape else _broadcast_view(a, shape))
             else:
                 if const is not None:
                     a = full(shape, a, t)

@@ -10,6 +10,7 @@
 #include <deque>
 #include <map>
 #include <stdexcept>
+#include <unordered_set>
 
 namespace bohrium {
 namespace jitk {
@@ -786,13 +787,16 @@ std::vector<LoopB> get_kernel_list(const std::vector<bh_instruction *> &instr_li
 std::vector<bh_instruction *> remove_non_computed_system_instr(std::vector<bh_instruction> &instr_list,
                                                               std::set<bh_base *> &frees) {
     std::vector<bh_instruction *> ret;
-    std::set<const bh_base *> computes;
+    ret.reserve(instr_list.size());
+    // (hash sets: flushes of tens of thousands of instructions pass through here)
+    std::unordered_set<const bh_base *> computes;
+    computes.reserve(2 * instr_list.size());
     for (bh_instruction &instr : instr_list) {
         if (instr.opcode == BH_FREE && computes.count(instr.operand[0].base) == 0) {
             frees.insert(instr.operand[0].base);
         } else if (!(instr.opcode == BH_NONE || instr.opcode == BH_TALLY)) {
-            auto b = allBases(instr);
-            computes.insert(b.begin(), b.end());
+            for (const bh_view &v : instr.operand)
+                if (!v.isConstant()) computes.insert(v.base);
             ret.push_back(&instr);
         }
     }
@@ -800,7 +804,8 @@ std::vector<bh_instruction *> remove_non_computed_system_instr(std::vector<bh_in
 }
 
 void set_constructor_flag(std::vector<bh_instruction *> &instr_list) {
-    std::set<bh_base *> constructed;
+    std::unordered_set<const bh_base *> constructed;
+    constructed.reserve(2 * instr_list.size());
     for (bh_instruction *instr : instr_list) {
         instr->constructor = false;
         for (size_t o = 0; o < instr->operand.size(); ++o) {

@@ -554,10 +554,21 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
                                 std::unordered_map<const bh_base *, int> &base2ord) const {
     std::string key;
     key.reserve(instrs.size() * 96);
+    base2ord.reserve(instrs.size());
     auto put = [&](int64_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
     // Equal constants share a kernel parameter: the key holds, for each constant, the position of the
     // first constant of equal type and value in the flush.
-    std::unordered_map<std::string, int64_t> first_equal;
+    struct ConstKey {
+        uint64_t lo, hi;
+        int type;
+        bool operator==(const ConstKey &o) const { return lo == o.lo && hi == o.hi && type == o.type; }
+    };
+    struct ConstHash {
+        size_t operator()(const ConstKey &k) const {
+            return std::hash<uint64_t>()(k.lo * 0x9e3779b97f4a7c15ull ^ k.hi ^ (static_cast<uint64_t>(k.type) << 56));
+        }
+    };
+    std::unordered_map<ConstKey, int64_t, ConstHash> first_equal;
     int64_t nconst = 0;
     for (const bh_instruction *i : instrs) {
         put(i->opcode);
@@ -570,9 +581,10 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
                 // A sweep's constant is its axis, which shapes the loop nest.
                 if (bh_opcode_is_sweep(i->opcode)) put(i->constant.get_int64());
                 if (i->constant.type != bh_type::R123) {
-                    std::string v(reinterpret_cast<const char *>(&i->constant.value), bh_type_size(i->constant.type));
-                    v.push_back(static_cast<char>(i->constant.type));
-                    put(first_equal.emplace(std::move(v), nconst).first->second);
+                    uint64_t w[2] = {0, 0};
+                    std::memcpy(w, &i->constant.value,
+                                std::min<size_t>(sizeof(w), static_cast<size_t>(bh_type_size(i->constant.type))));
+                    put(first_equal.emplace(ConstKey{w[0], w[1], static_cast<int>(i->constant.type)}, nconst).first->second);
                 }
                 ++nconst;
                 continue;
@@ -710,11 +722,13 @@ void EngineHip::handleExecution(BhIR *bhir) {
         for (const TwinUpdate &u : exchanges) twin_full_copies += u.full_copy ? 1 : 0;
     }
     jitk::set_constructor_flag(instr_list);
+    stats.time_prepare += seconds_since(t_total);
 
     auto t = std::chrono::steady_clock::now();
     std::vector<bh_base *> ord2base;
     std::unordered_map<const bh_base *, int> base2ord;
     const std::string key = flushKey(instr_list, ord2base, base2ord);
+    stats.time_key += seconds_since(t);
     auto hit = flush_cache.find(key);
     static const bool debug = std::getenv("BH_HIP_DEBUG") != nullptr;
     if (hit == flush_cache.end()) {
@@ -795,6 +809,7 @@ void EngineHip::handleExecution(BhIR *bhir) {
             for (int o : ck.free_ordinals) release(ord2base[o]);
         }
     }
+    const auto t_finish = std::chrono::steady_clock::now();
     // The updated arrays take the twins' buffers (their new values); the twins keep the old ones.
     for (const TwinUpdate &u : exchanges) {
         exchangeTwin(u);
@@ -822,6 +837,7 @@ void EngineHip::handleExecution(BhIR *bhir) {
             handOutHost(b);
         }
     }
+    stats.time_finish += seconds_since(t_finish);
     stats.time_total += seconds_since(t_total);
 }
 
@@ -987,6 +1003,10 @@ void EngineHip::resolveTuning() {
 }
 
 void EngineHip::handleExtmethod(BhIR *bhir) {
+    // Extension methods have the opcodes after the built-in ones: none, nothing to split.
+    bool any = false;
+    for (const bh_instruction &instr : bhir->instr_list) any |= instr.opcode > BH_MAX_OPCODE_ID;
+    if (!any) return;
     std::vector<bh_instruction> instr_list;
     for (bh_instruction &instr : bhir->instr_list) {
         auto ext = comp.extmethods.find(instr.opcode);
@@ -1016,10 +1036,10 @@ void EngineHip::handleExtmethod(BhIR *bhir) {
         } else if (comp.child_extmethods.count(instr.opcode)) {
             throw std::runtime_error("hip engine: extension method is only known to a child component");
         } else {
-            instr_list.push_back(instr);
+            instr_list.push_back(std::move(instr));
         }
     }
-    bhir->instr_list = instr_list;
+    bhir->instr_list = std::move(instr_list);
 }
 
 void *EngineHip::getMemoryPointer(bh_base &base, bool copy2host, bool force_alloc, bool nullify) {
@@ -1185,7 +1205,9 @@ std::string EngineHip::statistics() const {
     ss << "h2d_bytes: " << stats.h2d_bytes << "\nd2h_bytes: " << stats.d2h_bytes << "\n";
     ss << "time_fuse: " << stats.time_fuse << "\ntime_codegen: " << stats.time_codegen
        << "\ntime_compile: " << stats.time_compile << "\ntime_launch: " << stats.time_launch
-       << "\ntime_copy: " << stats.time_copy << "\ntime_total: " << stats.time_total << "\n";
+       << "\ntime_copy: " << stats.time_copy << "\ntime_total: " << stats.time_total
+       << "\ntime_prepare: " << stats.time_prepare << "\ntime_key: " << stats.time_key
+       << "\ntime_finish: " << stats.time_finish << "\n";
     for (const auto &kv : stats.device_ms_by_strategy) ss << "device_ms_" << kv.first << ": " << kv.second << "\n";
     ss << "flush_cache_hits: " << flush_cache_hits << "\nflush_cache_misses: " << flush_cache_misses << "\n";
     ss << "missed_frees: " << missed_frees << "\n";
@@ -1285,8 +1307,9 @@ public:
         // in place through its twin alternates between two buffers).  A replayed iteration also
         // repeats what its flush did to engine state besides launching (write counts, twin
         // exchanges).
+        const bool with_ext = has_extmethods(*bhir);
         const bool graphable = engine.graph_repeats && !engine.profile && !engine.dry_run && cond == nullptr &&
-                               nrepeats >= 4 && !has_slides(*bhir) && !has_extmethods(*bhir);
+                               nrepeats >= 4 && !has_slides(*bhir) && !with_ext;
         struct Iteration {
             std::vector<EngineHip::IssuedLaunch> launches;
             bool quiet;
@@ -1333,6 +1356,9 @@ public:
             const std::vector<uint64_t> q0 = engine.quietToken();
             cur.clear();
             if (rec) engine.recording = &cur;
+            // Running extension methods takes them out of the list: a later iteration runs them again.
+            std::vector<bh_instruction> body;
+            if (with_ext && i + 1 < nrepeats) body = bhir->instr_list;
             try {
                 engine.handleExtmethod(bhir);
                 engine.handleExecution(bhir);
@@ -1341,6 +1367,7 @@ public:
                 throw;
             }
             engine.recording = nullptr;
+            if (with_ext && i + 1 < nrepeats) bhir->instr_list = std::move(body);
             EngineHip::FlushEffects fx = engine.takeEffects();
             if (rec) {
                 hist.push_back({cur, engine.quietToken() == q0, std::move(fx)});

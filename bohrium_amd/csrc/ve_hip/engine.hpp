@@ -17,6 +17,7 @@
 #include <set>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 #include <bohrium/bh_component.hpp>
 #include <bohrium/jitk/fuser.hpp>
@@ -60,6 +61,9 @@ struct EngineStats {
     uint64_t flushes = 0, kernels = 0, launches = 0, h2d_bytes = 0, d2h_bytes = 0;
     std::map<std::string, uint64_t> launches_by_strategy;
     double time_fuse = 0, time_codegen = 0, time_compile = 0, time_launch = 0, time_copy = 0, time_total = 0;
+    /* Host phases of a flush: system instructions, frees and rewrites before the plan lookup; the
+     * lookup key; after the launches (releases, twin exchanges, error check, syncs). */
+    double time_prepare = 0, time_key = 0, time_finish = 0;
     /* Per-launch device time (ms) keyed by strategy, recorded when profiling is enabled. */
     std::map<std::string, double> device_ms_by_strategy;
 };
@@ -209,7 +213,7 @@ private:
         flush_written.insert(base);
     }
     /* Bases released during the current flush; a BH_FREE that no kernel released is caught with it. */
-    std::set<const bh_base *> released_now;
+    std::unordered_set<const bh_base *> released_now;
     DevicePool pool;
     unsigned char *reduce_scratch = nullptr;
     uint64_t reduce_scratch_bytes = 0;

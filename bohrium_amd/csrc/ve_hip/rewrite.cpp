@@ -42,6 +42,11 @@ bh_instruction reduce_instr(bh_opcode op, const bh_view &out, const bh_view &in,
 
 int unflatten_full_reductions(std::vector<bh_instruction *> &list, RewriteStore &store) {
     int count = 0;
+    // Only a full write of an N-D view can be restated: none, nothing to do (one quick pass).
+    bool nd = false;
+    for (const bh_instruction *i : list)
+        nd |= i->opcode != BH_FREE && !i->operand.empty() && !i->operand[0].isConstant() && i->operand[0].ndim >= 2;
+    if (!nd) return 0;
     for (bool changed = true; changed;) {
         changed = false;
         std::set<const bh_base *> freed;

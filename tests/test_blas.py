@@ -337,3 +337,25 @@ def test_trmm_alpha_and_strided_b():
     got = big.asnumpy()
     _close(got[:, ::2], u @ b, np.abs(u) @ np.abs(b), np.float64)
     assert np.all(got[:, 1::2] == 0)
+
+
+@pytest.mark.gpu
+def test_gemm_inside_a_repeated_body():
+    """An extension method inside a do_while body runs in every iteration: X := A X, three times in
+    one flush (the engine takes extension methods out of a flush's instruction list to run them,
+    so a repeated body must get them back for the next iteration).  Bound: each product of 64
+    terms rounds within 64 * 2^-53 of the exact one, relative to |A| |X|."""
+    import bohrium_amd as bh
+    a = _rand((64, 64), np.float64, 5) * 0.1
+    x0 = _rand((64, 32), np.float64, 6)
+    A, X = bh.array(a), bh.array(x0)
+
+    def body():
+        X[...] = bh.blas.gemm(A, X)
+
+    bh.do_while(body, 3)
+    got = X.asnumpy()
+    want = a @ (a @ (a @ x0))
+    mag = np.abs(a) @ (np.abs(a) @ (np.abs(a) @ np.abs(x0)))
+    assert np.all(np.abs(got - want) <= 3 * 64 * 2.0 ** -53 * mag + 1e-300)
+    assert not np.allclose(got, a @ x0)  # more than one iteration ran
