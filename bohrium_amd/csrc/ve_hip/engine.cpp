@@ -163,7 +163,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     graph_repeats = cfg.defaultGet<bool>("graph_repeats", true);
     batch_min = cfg.defaultGet<int>("batch_min", 8);
     batch_scan_group = std::max(1, cfg.defaultGet<int>("batch_scan_group", 8));
-    batch_xcds = std::max(1, cfg.defaultGet<int>("batch_xcds", 1));
+    batch_xcds = std::max(1, cfg.defaultGet<int>("batch_xcds", 8));
     persistent = cfg.defaultGet<bool>("batch_persistent", false);
     persist_items = std::max(0, cfg.defaultGet<int>("persist_items", 16384));
     persist_grid = std::max(0, cfg.defaultGet<int>("persist_grid", 0));

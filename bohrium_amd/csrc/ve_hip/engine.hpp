@@ -138,9 +138,10 @@ public:
     int batch_min = 8;
     /* Batched scans: items whose tiles are interleaved with each other (1: each item's tiles in a row). */
     int batch_scan_group = 8;
-    /* Batched scans: XCDs the workgroups go to round robin (all tiles of an item on one; 1, the
-     * default: no placement, which measured the same on the batched-arrays workload). */
-    int batch_xcds = 1;
+    /* Batched scans: XCDs the workgroups go to round robin (all tiles of an item on one XCD, whose L2
+     * then serves the item's gathers: a quarter of the HBM reads on the batched-arrays workload, at
+     * the same time; 1: no placement). */
+    int batch_xcds = 8;
     uint64_t batch_launches = 0, batched_items = 0;
     void setBatchMin(int n) {
         batch_min = n;
