@@ -135,7 +135,8 @@ def cpu_secondary_baselines():
     t = sum(prog.run_next_group() for _ in range(sweeps))
     alg = 8192 * 8192 * 4 + 8190 * 8190 * 4
     out["jacobi"] = {"value": round(alg * sweeps / t / 1e9, 3), "unit": "GB/s", "ms_per_sweep": round(t / sweeps * 1e3, 3),
-                     "sample": "%d sweeps of the 8192x8192 float32 grid (stencil + max|delta| + copy-back)" % sweeps}
+                     "sample": "%d sweeps of the 8192x8192 float32 grid (stencil + max|delta| + copy-back, "
+                               "the program as written)" % sweeps}
     sizes = workloads.batched_sizes(4096)[:512]
     with api.record_only() as r:
         xs, idxs = workloads.batched_inputs(sizes)
@@ -215,12 +216,13 @@ def jacobi_secondary(bh, workloads):
     dt = time.perf_counter() - t0
     replays = int(bh.message("graph replays")) - before
     alg = n * n * 4 + (n - 2) * (n - 2) * 4
-    return {"workload": "jacobi fp32 %dx%d, %d sweeps (stencil + max|delta| + copy-back), do_while replayed as a HIP graph"
+    return {"workload": "jacobi fp32 %dx%d, %d sweeps (c[...] = stencil(c) with max|delta|: the stencil writes the "
+                        "grid's twin buffer and the buffers are exchanged), do_while replayed as a HIP graph"
                         % (n, n, iters),
             "ms_per_sweep": round(dt * 1e3 / iters, 4), "effective_GBs": round(alg * iters / dt / 1e9, 1),
             "algorithmic_bytes_per_sweep": alg, "graph_replays": replays,
             "kernel_ms": {"stencil+max (reduce)": round(sum(red) / max(1, len(red)), 4),
-                          "copy-back (map)": round(sum(mp) / max(1, len(mp)), 4)},
+                          "other maps": round(sum(mp) / max(1, len(mp)), 4)},
             "last_delta": float(d)}
 
 
