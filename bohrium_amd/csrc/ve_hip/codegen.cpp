@@ -473,7 +473,11 @@ private:
          * are advanced by the grid stride instead of being divided out of f_ for every element. */
         std::string inner;
     };
+    /* Grid-stride over [0, tot).  With `parts`, the range is cut into min(8, bh_nblk) contiguous
+     * parts, part x walked by the workgroups b with b mod 8 = x (the workgroups one XCD runs), so a
+     * stencil's neighbouring rows are read by the workgroups of one L2. */
     void grid_stride(const std::string &tot, int unroll, const Element &e);
+    bool parts = false;
     std::vector<const bh_view *> preloadable(const LoopB &l);
     void emit_map();
     void emit_row_reduce();
@@ -806,9 +810,46 @@ std::vector<const bh_view *> Writer::preloadable(const LoopB &l) {
 // Full reductions with at most this many instructions in their loop count as light (half the grid).
 constexpr size_t kLightReduceInstrs = 16;
 
-void Writer::grid_stride(const std::string &tot, int unroll, const Element &e) {
-    body << "  const int64_t st_ = (int64_t)bh_nblk * BH_WG;\n";
-    body << "  int64_t g_ = (int64_t)bh_bid * BH_WG + threadIdx.x;\n";
+namespace {
+// A base read through two or more different views in `l` (a stencil: neighbouring elements of one
+// array read at once).
+bool shifted_reads(const LoopB &l) {
+    std::map<const bh_base *, std::set<std::vector<int64_t> > > views;
+    for (const InstrPtr &i : jitk::allInstr(l)) {
+        for (size_t o = 1; o < i->operand.size(); ++o) {
+            const bh_view &v = i->operand[o];
+            if (v.isConstant()) continue;
+            std::vector<int64_t> key{v.start, v.ndim};
+            for (int64_t d = 0; d < v.ndim; ++d) {
+                key.push_back(v.shape[d]);
+                key.push_back(v.stride[d]);
+            }
+            auto &set = views[v.base];
+            set.insert(key);
+            if (set.size() > 1) return true;
+        }
+    }
+    return false;
+}
+} // namespace
+
+void Writer::grid_stride(const std::string &tot_in, int unroll, const Element &e) {
+    std::string tot = tot_in;
+    if (parts) {
+        // Part x of X = min(8, bh_nblk) holds [x * pt_, (x + 1) * pt_); its workgroups are
+        // b = x, x + X, ... (q_ or q_ + 1 of them), each taking every whole workgroup's worth in turn.
+        body << "  const unsigned px_ = bh_nblk < BH_XCDS ? bh_nblk : BH_XCDS;\n";
+        body << "  const unsigned pw_ = bh_nblk / px_ + (bh_bid % px_ < bh_nblk % px_ ? 1u : 0u);\n";
+        body << "  const int64_t pt_ = ((" << tot << " + px_ - 1) / px_ + BH_WG - 1) / BH_WG * BH_WG;\n";
+        body << "  const int64_t p0_ = (int64_t)(bh_bid % px_) * pt_;\n";
+        body << "  const int64_t pe_ = p0_ + pt_ < " << tot << " ? p0_ + pt_ : " << tot << ";\n";
+        body << "  const int64_t st_ = (int64_t)pw_ * BH_WG;\n";
+        body << "  int64_t g_ = p0_ + (int64_t)(bh_bid / px_) * BH_WG + threadIdx.x;\n";
+        tot = "pe_";
+    } else {
+        body << "  const int64_t st_ = (int64_t)bh_nblk * BH_WG;\n";
+        body << "  int64_t g_ = (int64_t)bh_bid * BH_WG + threadIdx.x;\n";
+    }
     if (unroll > 1) {
         const std::vector<const bh_view *> reads =
             e.body_loop != nullptr ? preloadable(*e.body_loop) : std::vector<const bh_view *>();
@@ -1072,7 +1113,9 @@ void Writer::emit_map() {
             body << std::string(2 * ind + 2 * r, ' ') << "}\n";
         }
     };
+    parts = cfg.xcd_parts && shifted_reads(l0);
     grid_stride("tot_", flat ? cfg.map_unroll : 1, e);
+    parts = false;
     write_back(unit, ksc, 1);
     kp.grid = std::max<uint64_t>(1, std::min<uint64_t>((total + kp.block - 1) / kp.block, cfg.max_grid));
 }
@@ -1246,7 +1289,9 @@ void Writer::emit_full_reduce() {
             emit_blocks(*l1, sc, ind);
             write_back(*l1, sc, ind);
         };
+        parts = cfg.xcd_parts && shifted_reads(l0);
         grid_stride("tot_", cfg.reduce_unroll, e);
+        parts = false;
     } else {
         Element e;
         e.outer = &ksc;
@@ -1259,7 +1304,9 @@ void Writer::emit_full_reduce() {
             emit_blocks(l0, sc, ind);
             write_back(l0, sc, ind);
         };
+        parts = cfg.xcd_parts && shifted_reads(l0);
         grid_stride("n" + std::to_string(lid.at(&l0)), l0.isInnermost() ? cfg.reduce_unroll : 1, e);
+        parts = false;
     }
     body << "  __shared__ __attribute__((aligned(16))) unsigned char bh_lds_[BH_WAVES * 16];\n";
     auto op_lambda = [&](const Acc &a) {

@@ -106,6 +106,10 @@ struct WriterConfig {
     bool math64 = true;
     /* Grid-stride skeletons renumber their workgroups so that each XCD runs a contiguous range. */
     bool xcd_remap = false;
+    /* Grid-stride skeletons of stencils (a base read through several views) walk the iteration space
+     * in 8 contiguous parts, one per XCD (workgroup b runs on XCD b mod 8): a row's neighbours are
+     * read under the same L2. */
+    bool xcd_parts = true;
     uint32_t epoch = 1;      // look-back flag generation (changes every launch)
 };
 

@@ -157,6 +157,7 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     wcfg.row_reduce = cfg.defaultGet<bool>("row_reduce", true);
     wcfg.waves_per_eu = std::max(0, cfg.defaultGet<int>("waves_per_eu", 0));
     wcfg.xcd_remap = cfg.defaultGet<bool>("xcd_remap", false);
+    wcfg.xcd_parts = cfg.defaultGet<bool>("xcd_parts", true);
     unflatten = cfg.defaultGet<bool>("unflatten_reductions", true);
     resident = cfg.defaultGet<bool>("resident", true);
     track_host_writes = cfg.defaultGet<bool>("host_write_tracking", true);
@@ -178,8 +179,11 @@ EngineHip::EngineHip(component::ComponentVE &comp) : comp(comp) {
     tuner.samples = std::max(1, cfg.defaultGet<int>("autotune_samples", 3));
     tuner.min_items = cfg.defaultGet<uint64_t>("autotune_min_items", 1u << 20);
     tuner.min_gain = cfg.defaultGet<double>("autotune_min_gain", 0.02);
-    tuner.mode = cfg.defaultGet<std::string>("autotune_mode", "measure") == "model" ? Tuner::Mode::MODEL
-                                                                                    : Tuner::Mode::MEASURE;
+    {
+        const std::string m = cfg.defaultGet<std::string>("autotune_mode", "measure");
+        tuner.mode = m == "model" ? Tuner::Mode::MODEL : m == "prune" ? Tuner::Mode::PRUNE : Tuner::Mode::MEASURE;
+        tuner.prune_keep = std::max(1, cfg.defaultGet<int>("autotune_prune_keep", 2));
+    }
     if (!cache_dir.empty() && cfg.defaultGet<bool>("autotune_persist", true)) {
         tuner.path = cache_dir + "/autotune_" + cfg.defaultGet<std::string>("dry_run_arch", "gfx950") + ".txt";
         tuner.load();
@@ -904,6 +908,27 @@ void EngineHip::tuneKernel(CachedKernel &ck, const jitk::LoopB &kernel,
         ++tuner.model_picks;
         return;
     }
+    if (tuner.mode == Tuner::Mode::PRUNE && !t->predicted_ms.empty() &&
+        t->variants.size() > static_cast<size_t>(tuner.prune_keep) + 1) {
+        // The configured knobs and the model's best-predicted candidates go on trial; the others not.
+        std::vector<size_t> order;
+        for (size_t c = 1; c < t->variants.size(); ++c) order.push_back(c);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return t->predicted_ms[a] < t->predicted_ms[b]; });
+        std::vector<size_t> keep = {0};
+        keep.insert(keep.end(), order.begin(), order.begin() + tuner.prune_keep);
+        std::sort(keep.begin(), keep.end());
+        auto pick = [&](auto &v) {
+            std::remove_reference_t<decltype(v)> kept;
+            for (size_t c : keep) kept.push_back(std::move(v[c]));
+            v.swap(kept);
+        };
+        tuner.pruned += t->variants.size() - keep.size();
+        pick(t->knobs);
+        pick(t->variants);
+        pick(t->features);
+        pick(t->predicted_ms);
+    }
     t->ms.resize(t->variants.size());
     t->inflight.assign(t->variants.size(), 0);
     ck.trial = std::move(t);
@@ -1218,7 +1243,7 @@ std::string EngineHip::statistics() const {
     ss << "batch_launches: " << batch_launches << "\nbatched_items: " << batched_items << "\n";
     ss << "persist_launches: " << persist_launches << "\npersist_fallbacks: " << persist_fallbacks << "\n";
     ss << "twin_exchanges: " << twin_exchanges << "\ntwin_full_copies: " << twin_full_copies << "\n";
-    ss << "autotune_model_picks: " << tuner.model_picks << "\n";
+    ss << "autotune_model_picks: " << tuner.model_picks << "\nautotune_pruned: " << tuner.pruned << "\n";
     ss << "autotune_trials: " << tuner.trials << "\nautotune_decisions: " << tuner.decisions
        << "\nautotune_kept_configured: " << tuner.kept_default << "\nautotune_table_hits: " << tuner.table_hits
        << "\nautotune_saved_ms: " << tuner.saved_ms << "\n";
@@ -1457,7 +1482,7 @@ public:
                 const std::string k = kv.substr(0, eq);
                 const int v = std::stoi(kv.substr(eq + 1));
                 if (v < 0 || (v < 1 && k != "row_reduce" && k != "waves_per_eu" && k != "resident_grid" &&
-                              k != "math64" && k != "xcd_remap"))
+                              k != "math64" && k != "xcd_remap" && k != "xcd_parts"))
                     throw std::invalid_argument("hip engine: codegen setting must be >= 1: " + kv);
                 if (k == "max_grid") {
                     engine.codegenConfig().max_grid = v;
@@ -1471,6 +1496,8 @@ public:
                     engine.codegenConfig().math64 = v != 0;
                 } else if (k == "xcd_remap") {
                     engine.codegenConfig().xcd_remap = v != 0;
+                } else if (k == "xcd_parts") {
+                    engine.codegenConfig().xcd_parts = v != 0;
                 } else if (k == "reduce_max_grid") {
                     engine.codegenConfig().reduce_max_grid = v;
                 } else if (k == "reduce_min_items") {
@@ -1490,7 +1517,8 @@ public:
                << " reduce_max_grid=" << w.reduce_max_grid << " reduce_min_items=" << w.reduce_min_items
                << " scan_items=" << w.scan_items << " row_reduce=" << (w.row_reduce ? 1 : 0)
                << " waves_per_eu=" << w.waves_per_eu << " resident_grid=" << (w.resident_grid ? 1 : 0)
-               << " math64=" << (w.math64 ? 1 : 0) << " xcd_remap=" << (w.xcd_remap ? 1 : 0);
+               << " math64=" << (w.math64 ? 1 : 0) << " xcd_remap=" << (w.xcd_remap ? 1 : 0)
+               << " xcd_parts=" << (w.xcd_parts ? 1 : 0);
         } else if (msg.rfind("batch_min:", 0) == 0) {
             // Smallest group of same-code launches run as one batched launch (0: never); drops cached plans.
             engine.setBatchMin(std::stoi(msg.substr(10)));
@@ -1527,8 +1555,10 @@ public:
         } else if (msg.rfind("autotune_min_items:", 0) == 0) {
             engine.tuner.min_items = std::stoull(msg.substr(19));
             engine.dropPlans();
-        } else if (msg == "autotune_mode: measure" || msg == "autotune_mode: model") {
-            engine.tuner.mode = msg == "autotune_mode: model" ? Tuner::Mode::MODEL : Tuner::Mode::MEASURE;
+        } else if (msg == "autotune_mode: measure" || msg == "autotune_mode: model" || msg == "autotune_mode: prune") {
+            engine.tuner.mode = msg == "autotune_mode: model"   ? Tuner::Mode::MODEL
+                                : msg == "autotune_mode: prune" ? Tuner::Mode::PRUNE
+                                                                : Tuner::Mode::MEASURE;
             engine.dropPlans();
         } else if (msg == "autotune samples") {
             // Timed trial launches since the last read: "f=<block and launch features> ms=<device ms>".

@@ -45,9 +45,11 @@ class Tuner {
 public:
     bool enabled = true;
     /* measure: trials decide (the default); model: the launch-shape model (bhcm_install_launch)
-     * picks the candidate with the lowest predicted time and no trial runs (measure when no model
-     * is installed). */
-    enum class Mode { MEASURE, MODEL } mode = Mode::MEASURE;
+     * picks the candidate with the lowest predicted time and no trial runs; prune: trials of the
+     * configured knobs and the model's `prune_keep` best-predicted candidates only (model and prune
+     * measure every candidate when no model is installed). */
+    enum class Mode { MEASURE, MODEL, PRUNE } mode = Mode::MEASURE;
+    int prune_keep = 2;
     int samples = 3;                  // timed launches per candidate
     uint64_t min_items = 1u << 20;    // smaller kernels keep the configured knobs
     double min_gain = 0.02;           // a candidate must beat the configured knobs by this fraction
@@ -70,7 +72,7 @@ public:
     void clear() { table.clear(); }
     std::string dump() const;
 
-    uint64_t trials = 0, decisions = 0, table_hits = 0, kept_default = 0, model_picks = 0;
+    uint64_t trials = 0, decisions = 0, table_hits = 0, kept_default = 0, model_picks = 0, pruned = 0;
     double saved_ms = 0;  // sum over decisions of (configured - chosen) per-launch device time
 
     /* Training data of the launch-shape model: block + launch features of each timed trial launch

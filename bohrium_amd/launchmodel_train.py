@@ -112,7 +112,17 @@ def judge(seeds, sides=(1024, 2048), extra=True, log=print):
     tot_default = sum(ms[0] for _, _, ms, _ in picks)
     log("%d kernels: model picks %.3f ms, measured best %.3f ms, configured knobs %.3f ms"
         % (len(picks), tot_model, tot_best, tot_default))
+    for k in (2, 3):
+        tot_k = sum(min(ms[0], min(ms[j] for j in top_k(pred, k))) for _, _, ms, pred in picks)
+        trials = sum(min(len(ms), k + 1) for _, _, ms, _ in picks)
+        log("  prune to the configured knobs + the model's best %d: %.3f ms (%d candidates measured, of %d)"
+            % (k, tot_k, trials, sum(len(ms) for _, _, ms, _ in picks)))
     return picks, tot_model, tot_best, tot_default
+
+
+def top_k(pred, k):
+    """Indices of the k candidates with the lowest predicted time."""
+    return [int(j) for j in np.argsort(np.asarray(pred))[:k]]
 
 
 def main(argv=None):
@@ -126,7 +136,8 @@ def main(argv=None):
     api.set_flush_threshold(10 ** 7)
     lo, hi = (int(v) for v in a.test.split(":"))
     if a.eval_only:
-        costmodel.CostModel.load(a.out).install_launch()
+        model = costmodel.CostModel.load(a.out)  # (installed only while this object lives)
+        model.install_launch()
         judge(range(lo, hi), extra=False)
         return
     tlo, thi = (int(v) for v in a.train.split(":"))
