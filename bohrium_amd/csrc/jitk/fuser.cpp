@@ -4,6 +4,7 @@
  * Boost.Graph); partition results can differ where the reference breaks ties
  * on Boost's edge order. */
 #include <bohrium/jitk/fuser.hpp>
+#include <bohrium/flat_ptr_map.hpp>
 
 #include <algorithm>
 #include <cassert>
@@ -788,15 +789,14 @@ std::vector<bh_instruction *> remove_non_computed_system_instr(std::vector<bh_in
                                                               std::set<bh_base *> &frees) {
     std::vector<bh_instruction *> ret;
     ret.reserve(instr_list.size());
-    // (hash sets: flushes of tens of thousands of instructions pass through here)
-    std::unordered_set<const bh_base *> computes;
-    computes.reserve(2 * instr_list.size());
+    // (flat hash sets: flushes of tens of thousands of instructions pass through here)
+    FlatPtrMap computes(instr_list.size());
     for (bh_instruction &instr : instr_list) {
-        if (instr.opcode == BH_FREE && computes.count(instr.operand[0].base) == 0) {
+        if (instr.opcode == BH_FREE && computes.find(instr.operand[0].base) == nullptr) {
             frees.insert(instr.operand[0].base);
         } else if (!(instr.opcode == BH_NONE || instr.opcode == BH_TALLY)) {
             for (const bh_view &v : instr.operand)
-                if (!v.isConstant()) computes.insert(v.base);
+                if (!v.isConstant()) computes.insert(v.base, 0);
             ret.push_back(&instr);
         }
     }
@@ -804,15 +804,13 @@ std::vector<bh_instruction *> remove_non_computed_system_instr(std::vector<bh_in
 }
 
 void set_constructor_flag(std::vector<bh_instruction *> &instr_list) {
-    std::unordered_set<const bh_base *> constructed;
-    constructed.reserve(2 * instr_list.size());
+    FlatPtrMap constructed(instr_list.size());
     for (bh_instruction *instr : instr_list) {
         instr->constructor = false;
         for (size_t o = 0; o < instr->operand.size(); ++o) {
             const bh_view &v = instr->operand[o];
             if (v.isConstant()) continue;
-            if (o == 0 && constructed.count(v.base) == 0) instr->constructor = true;
-            constructed.insert(v.base);
+            if (constructed.insert(v.base, 0).second && o == 0) instr->constructor = true;
         }
     }
 }

@@ -1,5 +1,7 @@
 #include "engine.hpp"
 
+#include <bohrium/flat_ptr_map.hpp>
+
 #include <sys/mman.h>
 #include <unistd.h>
 
@@ -554,12 +556,11 @@ size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> 
     return n;
 }
 
-std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
-                                std::unordered_map<const bh_base *, int> &base2ord) const {
-    std::string key;
-    key.reserve(instrs.size() * 96);
-    base2ord.reserve(instrs.size());
-    auto put = [&](int64_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base) const {
+    std::vector<int64_t> words;  // the key as 64-bit words (one string made at the end)
+    words.reserve(instrs.size() * 24);
+    FlatPtrMap base2ord(instrs.size());
+    auto put = [&](int64_t v) { words.push_back(v); };
     // Equal constants share a kernel parameter: the key holds, for each constant, the position of the
     // first constant of equal type and value in the flush.
     struct ConstKey {
@@ -593,17 +594,13 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
                 ++nconst;
                 continue;
             }
-            auto it = base2ord.find(v.base);
-            int o;
-            if (it == base2ord.end()) {
-                o = static_cast<int>(ord2base.size());
-                base2ord[v.base] = o;
+            const std::pair<int, bool> r = base2ord.insert(v.base, static_cast<int>(ord2base.size()));
+            const int o = r.first;
+            if (r.second) {
                 ord2base.push_back(v.base);
                 put(-2);
                 put(static_cast<int64_t>(v.base->dtype()));
                 put(v.base->nelem());
-            } else {
-                o = it->second;
             }
             put(o);
             // A sliding view's start changes every repeat; it is a launch argument, not structure.
@@ -615,7 +612,7 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
             }
         }
     }
-    return key;
+    return std::string(reinterpret_cast<const char *>(words.data()), words.size() * sizeof(int64_t));
 }
 
 void EngineHip::resolveProfile() {
@@ -730,13 +727,14 @@ void EngineHip::handleExecution(BhIR *bhir) {
 
     auto t = std::chrono::steady_clock::now();
     std::vector<bh_base *> ord2base;
-    std::unordered_map<const bh_base *, int> base2ord;
-    const std::string key = flushKey(instr_list, ord2base, base2ord);
+    const std::string key = flushKey(instr_list, ord2base);
     stats.time_key += seconds_since(t);
     auto hit = flush_cache.find(key);
     static const bool debug = std::getenv("BH_HIP_DEBUG") != nullptr;
     if (hit == flush_cache.end()) {
         ++flush_cache_misses;
+        std::unordered_map<const bh_base *, int> base2ord;  // the flush's base ordinals (inverse of ord2base)
+        for (size_t o = 0; o < ord2base.size(); ++o) base2ord.emplace(ord2base[o], static_cast<int>(o));
         std::vector<jitk::LoopB> kernels = fold_serial_tails(jitk::get_kernel_list(instr_list, fusion_config));
         // A kernel releases the bases freed in it when it ends: no later kernel may use them.
         {

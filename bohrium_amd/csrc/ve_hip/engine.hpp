@@ -346,8 +346,7 @@ private:
     bool errors_unchecked = false;  // a launch since the last error-flag check can raise an error
     uint64_t scratch_allocs = 0;
     std::map<std::string, void *> graph_cache;
-    std::string flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base,
-                         std::unordered_map<const bh_base *, int> &base2ord) const;
+    std::string flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base) const;
     void launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2base,
                     const std::vector<bh_instruction *> &instrs);
     /* Autotuner: set up a trial for a kernel just planned (or take the table's choice), run one
