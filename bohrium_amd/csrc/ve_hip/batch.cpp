@@ -307,7 +307,8 @@ bool EngineHip::launchPersist(PersistPlan &pp, CachedFlush &cf, const std::vecto
     unsigned a_nlevels = static_cast<unsigned>(pp.levels.size() - 1);
     unsigned *a_bar = reinterpret_cast<unsigned *>(pp.dev + bar_at);
     unsigned *a_err = counters + 1;
-    void *params[] = {&a_args, &a_tasks, &a_levels, &a_nlevels, &a_bar, &a_err};
+    unsigned a_spin = persist_spin;
+    void *params[] = {&a_args, &a_tasks, &a_levels, &a_nlevels, &a_bar, &a_err, &a_spin};
     if (bhhip_launch_cooperative(pp.fn, pp.grid, 256, params) != 0) {
         // (e.g. a grid larger than the device holds at once): the levels run instead, now and later
         if (ev0 != nullptr) bhhip_event_destroy(ev0);

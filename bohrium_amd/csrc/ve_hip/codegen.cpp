@@ -1880,7 +1880,7 @@ std::string persist_source(const std::vector<const KernelPlan *> &codes) {
     s << "\nextern \"C\" __global__ void __launch_bounds__(256) bh_kernel_persist("
       << "const unsigned char *__restrict__ bh_args, const uint4 *__restrict__ bh_tasks, "
          "const unsigned *__restrict__ bh_levels, const unsigned bh_nlevels, unsigned *__restrict__ bh_bar, "
-         "unsigned *__restrict__ bh_gerr) {\n"
+         "unsigned *__restrict__ bh_gerr, const unsigned bh_spin) {\n"
       << "  for (unsigned L_ = 0; L_ < bh_nlevels; ++L_) {\n"
       << "    const unsigned t1_ = __builtin_amdgcn_readfirstlane(bh_levels[L_ + 1u]);\n"
       << "    for (unsigned t_ = __builtin_amdgcn_readfirstlane(bh_levels[L_]) + blockIdx.x; t_ < t1_; t_ += gridDim.x) {\n"
@@ -1892,7 +1892,7 @@ std::string persist_source(const std::vector<const KernelPlan *> &codes) {
       << "      switch (c_) {\n" << cases.str() << "        default: break;\n      }\n"
       << "      __syncthreads();\n"
       << "    }\n"
-      << "    if (L_ + 1u < bh_nlevels) bh_grid_sync(bh_bar, (L_ + 1u) * gridDim.x, bh_gerr);\n"
+      << "    if (L_ + 1u < bh_nlevels) bh_grid_sync(bh_bar, (L_ + 1u) * gridDim.x, bh_gerr, bh_spin);\n"
       << "  }\n"
       << "  bh_grid_exit(bh_bar);\n"
       << "}\n";

@@ -1527,6 +1527,11 @@ public:
         } else if (msg == "batch_persistent: on" || msg == "batch_persistent: off") {
             // Batched schedules as one persistent launch, or one launch per level and group.
             engine.persistent = msg == "batch_persistent: on";
+        } else if (msg.rfind("persist_spin:", 0) == 0) {
+            // Polls of a persistent launch's grid-barrier wait before it raises the error flag.
+            engine.persist_spin = static_cast<unsigned>(std::stoul(msg.substr(13)));
+        } else if (msg == "batch_persistent") {
+            ss << (engine.persistent ? "on" : "off");
         } else if (msg.rfind("persist_items:", 0) == 0 || msg.rfind("persist_grid:", 0) == 0) {
             // Elements per task of grid-stride plans, or the cap on the persistent grid (0: none);
             // drops cached plans (and with them their persistent plans).

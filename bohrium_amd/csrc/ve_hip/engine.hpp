@@ -328,6 +328,9 @@ public:
      * own grid. */
     int persist_items = 16384;
     int persist_grid = 0;  // >0: at most this many workgroups in the persistent launch
+    /* Polls of a grid barrier's wait before it gives up and raises the error flag (default 2^22,
+     * seconds of waiting; a tiny value forces the error path from the host). */
+    unsigned persist_spin = 1u << 22;
     uint64_t persist_launches = 0, persist_fallbacks = 0;
 private:
     /* `grid_arg`: the value packed for a GRID_U32 argument (0: the plan's grid). */

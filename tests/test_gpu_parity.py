@@ -540,3 +540,30 @@ def test_full_reduce_with_sweeping_tail_blocks():
         assert abs(float(got[i]) - float(want[i])) <= 2 * bound, (i, float(got[i]), float(want[i]))
         np.testing.assert_array_equal(got[n + i], want[n + i])
         assert_parity(got[2 * n + i], want[2 * n + i], np.float32, rtol=1e-5, atol=1e-6)
+
+
+def test_persistent_grid_barrier_timeout_is_reported():
+    """A grid barrier of the persistent launch that gives up waiting raises an engine error instead of
+    hanging, and persistent launches are then switched off.  Forced from the host: with a limit of
+    zero polls every workgroup that reaches a barrier before the last one gives up at its first poll
+    (the kernel still runs to its end, with wrong results).  The next flush runs as level launches
+    and matches the oracle."""
+    from bohrium_amd import _bh_api as api
+    prog, n = _batched_prog(16)
+    api.set_flush_threshold(10 ** 7)
+    bh.message("batch_min: 8")
+    bh.message("batch_persistent: on")
+    bh.message("persist_spin: 0")
+    try:
+        with pytest.raises(Exception, match="grid barrier"):
+            bh.asnumpy_many(prog())
+        assert bh.message("batch_persistent") == "off"
+        bh.message("persist_spin: 4194304")
+        got, want = run_both(prog)
+    finally:
+        bh.message("persist_spin: 4194304")
+        bh.message("batch_persistent: off")
+        api.set_flush_threshold(1000)
+    for i in range(n):
+        np.testing.assert_array_equal(got[n + i], want[n + i])  # cumsums
+        assert_parity(got[2 * n + i], want[2 * n + i], np.float32, rtol=1e-5, atol=1e-6)  # gathers
