@@ -272,6 +272,39 @@ def batched_secondary(bh, workloads, api):
     return out
 
 
+def streaming_secondary(bh):
+    """Memory-bound fused blocks on 2^28 float32 elements (1 GiB): a map + sum, a map, and a max of |x - c|,
+    each after the autotuner's trials have settled (device time of the launches from HIP events)."""
+    import numpy as np
+    n = 1 << 28
+    x = bh.random.RandomState(1).random_sample(n).astype(np.float32)
+    x[:1].asnumpy()
+    out = {"workload": "float32 x of 2^28 elements (1 GiB), philox uniforms"}
+    cases = (("sum(x*2+1)", lambda: (x * 2.0 + 1.0).sum(), 4 * n),
+             ("z = x*2+1", lambda: x * 2.0 + 1.0, 8 * n),
+             ("max|x-0.5|", lambda: bh.ufunc("maximum").reduce(bh.ufunc("absolute")(x - 0.5), 0), 4 * n))
+    for label, fn, nbytes in cases:
+        def run():
+            r = fn()
+            (r[:1] if r.ndim else r).asnumpy()
+        for _ in range(30):
+            run()
+        bh.message("autotune")
+        run()
+        bh.message("profile: on")
+        launches(bh)
+        ts = []
+        for _ in range(5):
+            run()
+            ts.append(sum(float(l[3]) for l in launches(bh) if l[0] in ("reduce", "map")))
+        bh.message("profile: off")
+        t = min(ts)
+        out[label] = {"device_ms": round(t, 4), "algorithmic_bytes": nbytes, "GBs": round(nbytes / t / 1e6, 1),
+                      "frac_of_hbm_peak": round(nbytes / t / 1e6 / HBM_PEAK_GBS, 4)}
+    del x
+    return out
+
+
 def gemm_secondary(bh):
     """blas_gemm (the MFMA extension method) on 8192 x 8192 operands, fp64 and fp32: wall time per
     product between host syncs, operands resident."""
@@ -376,7 +409,7 @@ def main():
     if not args.no_secondary:
         del S, K, T
         secondary = {"mc_pi": mc_pi_secondary(bh, workloads), "jacobi": jacobi_secondary(bh, workloads),
-                     "batched": batched_secondary(bh, workloads, api)}
+                     "batched": batched_secondary(bh, workloads, api), "streaming": streaming_secondary(bh)}
         if args.gemm:
             secondary["blas_gemm"] = gemm_secondary(bh)
 
