@@ -1343,8 +1343,19 @@ void Writer::emit_full_reduce() {
         body << "  " << T << " " << a.var << ";\n";
         body << "  { " << op_lambda(a) << "\n";
         body << "    " << T << " v_ = " << literal(jitk::sweep_identity(a.instr->opcode, a.t)) << ";\n";
-        body << "    for (unsigned j_ = threadIdx.x; j_ < bh_nparts; j_ += BH_WG) v_ = op_(v_, *(const " << T << " *)"
-             << slot_addr(a, "j_") << ");\n";
+        // Sixteen partials per thread are loaded before any is folded, so their memory latencies overlap
+        // (one load in flight at a time made this one-workgroup kernel latency-bound: up to 4096
+        // partials take one round trip); each thread still folds j_ = tid, tid + BH_WG, ... in order.
+        body << "    for (unsigned j_ = threadIdx.x; j_ < bh_nparts; j_ += 16u * BH_WG) {\n";
+        body << "      " << T << " x_[16];\n";
+        body << "#pragma unroll\n";
+        body << "      for (unsigned u_ = 0; u_ < 16u; ++u_)\n";
+        body << "        if (j_ + u_ * BH_WG < bh_nparts) x_[u_] = *(const " << T << " *)"
+             << slot_addr(a, "(j_ + u_ * BH_WG)") << ";\n";
+        body << "#pragma unroll\n";
+        body << "      for (unsigned u_ = 0; u_ < 16u; ++u_)\n";
+        body << "        if (j_ + u_ * BH_WG < bh_nparts) v_ = op_(v_, x_[u_]);\n";
+        body << "    }\n";
         body << "    " << a.var << " = bh_block_reduce(v_, op_, (" << T << " *)bh_lds_);\n";
         body << "    __syncthreads(); }\n";
     }
