@@ -31,7 +31,7 @@ namespace {
 const char *kPreludeRef() {
     static const char *p =
 #include "kernel_prelude.inc"
-        "\n#define BH_MATH_FN __device__ __forceinline__\n"
+        "\n#define BH_MATH_FN __device__ __forceinline__\n#define BH_MATH_TABLE static __constant__ const\n"
 #include "kernel_math.inc"
         ;
     return p;

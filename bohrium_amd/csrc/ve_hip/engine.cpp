@@ -27,7 +27,7 @@ namespace vehip {
 namespace {
 const char *kPrelude =
 #include "kernel_prelude.inc"
-    "\n#define BH_MATH_FN __device__ __forceinline__\n"
+    "\n#define BH_MATH_FN __device__ __forceinline__\n#define BH_MATH_TABLE static __constant__ const\n"
 #include "kernel_math.inc"
     ;
 

@@ -21,6 +21,7 @@ DRIVER = r"""
 #include <vector>
 using std::fma; using std::rint; using std::frexp; using std::ldexp; using std::fmin; using std::fmax;
 #define BH_MATH_FN static inline
+#define BH_MATH_TABLE static const
 %s
 int main(int argc, char **argv) {
     FILE *f = std::fopen(argv[1], "rb");
@@ -106,9 +107,12 @@ def test_special_values(host):
 
 @pytest.mark.parametrize("old,new,fn", [
     ("    r = fma(-n, ln2_lo, r);\n", "", "exp"),                          # the tail of ln2 dropped
-    ("p = fma(p, r, 1.0 / 479001600.0);", "p = fma(p, r, 0.0);", "exp"),  # the 1/12! term dropped
-    ("p = fma(p, s, 1.0 / 9.0);", "p = fma(p, s, 1.0 / 8.0);", "log"),    # a wrong series coefficient
-    ("m < 0.70710678118654752440", "m < 0.5", "log"),                    # no split at sqrt(1/2)
+    ("double q = c11;", "double q = 0.0;", "exp"),                            # the r^11 term dropped
+    ("q = fma(q, r, c7);", "q = fma(q, r, -c7);", "exp"),                    # a sign error
+    ("p = fma(p, r, 1.0 / 5.0);", "p = fma(p, r, 1.0 / 4.0);", "log"),       # a wrong series coefficient
+    ("m < 0.70703125", "m < 0.5", "log"),                                    # no move of m into one range
+    (">> 45) & 127u", ">> 44) & 127u", "log"),                               # the table index off by a bit
+    ("const double r = fma(m, t[0], -1.0);", "const double r = m * t[0] - 1.0;", "log"),  # r rounded twice
 ])
 def test_a_planted_mistake_is_caught(old, new, fn, monkeypatch, tmp_path):
     """The 2-ulp sweep has teeth: each of these edits to the source fails it."""
@@ -139,3 +143,30 @@ def test_device_matches_host_bit_for_bit(host):
     assert not bad.any(), "exp differs at %r" % x[bad][:5]
     bad = ~((dl == l) | (np.isnan(dl) & np.isnan(l)))
     assert not bad.any(), "log differs at %r" % x[bad][:5]
+
+
+def test_constants_are_the_generators():
+    """The exp coefficients and the log table in the source are the ones gen_kernel_math.py derives
+    (interpolation at Chebyshev nodes and -log(c_j) in 60-digit arithmetic), and each table entry
+    satisfies what the log relies on: c_j is 1 next to m = 1 and has at most 9 significant bits, and
+    head + tail is -log(c_j) to within 2^-100 relative."""
+    mp = pytest.importorskip("mpmath")
+    import importlib.util
+    path = os.path.join(ROOT, "bohrium_amd", "csrc", "ve_hip", "gen_kernel_math.py")
+    spec = importlib.util.spec_from_file_location("gen_kernel_math", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    text = open(SRC).read()
+    expected = gen.text()
+    for line in expected.splitlines():
+        if line.startswith("    const double c") or line.startswith("    {"):
+            assert line in text, "kernel_math.inc lacks %r" % line
+    rows = gen.log_table()
+    assert rows[0][0] == 1.0 and rows[127][0] == 1.0 and rows[0][1] == 0.0 and rows[127][1] == 0.0
+    mp.mp.dps = 60
+    for c, hi, lo in rows:
+        mant = int(mp.ldexp(mp.mpf(c), 8 - int(mp.floor(mp.log(c, 2)))))
+        assert mp.mpf(mant) == mp.ldexp(mp.mpf(c), 8 - int(mp.floor(mp.log(c, 2)))), "c = %r has > 9 bits" % c
+        if c != 1.0:
+            assert abs((mp.mpf(hi) + mp.mpf(lo)) / -mp.log(c) - 1) < mp.mpf(2) ** -100
+    assert gen.max_reduced_argument(rows) <= mp.mpf(2) ** -7
