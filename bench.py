@@ -176,6 +176,18 @@ def autotune_trials(bh):
     return 0.0
 
 
+def settle(bh, run, limit):
+    """Calls run() until a call starts no autotuner trial launch (at most `limit` calls), then settles
+    the finished trials; returns the number of calls."""
+    for k in range(1, limit + 1):
+        before = autotune_trials(bh)
+        run()
+        if autotune_trials(bh) == before:
+            break
+    bh.message("autotune")
+    return k
+
+
 def launches(bh):
     return [l.split() for l in bh.message("launches").splitlines() if l.strip()]
 
@@ -185,8 +197,7 @@ def mc_pi_secondary(bh, workloads):
     FULL_REDUCE launch and its finalize); compute-bound, reported as points per second."""
     n, reps = 10_000_000, 20
     count = int(workloads.mc_pi(n).asnumpy())
-    for _ in range(30):  # lets the autotuner settle the kernel's launch shape
-        workloads.mc_pi(n).asnumpy()
+    settle(bh, lambda: workloads.mc_pi(n).asnumpy(), 100)  # the autotuner's trials of the launch shape
     bh.message("profile: on")
     launches(bh)
     for _ in range(3):
@@ -209,9 +220,10 @@ def jacobi_secondary(bh, workloads):
     """2-D Jacobi fp32 8192^2, 200 sweeps replayed as one graph per sweep."""
     n, iters = 8192, 200
     G = workloads.jacobi_grid(n)
-    workloads.jacobi_repeat(G, 40).asnumpy()  # includes the autotuner's trials of both kernels
-    bh.message("autotune")  # settle the trials
-    workloads.jacobi_repeat(G, 40).asnumpy()  # the repeat's graphs for the chosen launch shapes
+    # Warm-up in 40-sweep runs until a run starts no autotuner trial (at most 8 runs), then one more
+    # run for the repeat's graphs of the chosen launch shapes.
+    settle(bh, lambda: workloads.jacobi_repeat(G, 40).asnumpy(), 8)
+    workloads.jacobi_repeat(G, 40).asnumpy()
     bh.message("profile: on")
     launches(bh)
     workloads.jacobi_repeat(G, 3).asnumpy()
@@ -287,9 +299,7 @@ def streaming_secondary(bh):
         def run():
             r = fn()
             (r[:1] if r.ndim else r).asnumpy()
-        for _ in range(30):
-            run()
-        bh.message("autotune")
+        settle(bh, run, 100)
         run()
         bh.message("profile: on")
         launches(bh)

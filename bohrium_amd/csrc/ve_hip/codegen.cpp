@@ -472,6 +472,9 @@ private:
         /* A 2-D iteration (i0 outer, i1 inner of this extent): without unrolling, the two indices
          * are advanced by the grid stride instead of being divided out of f_ for every element. */
         std::string inner;
+        /* decode is i0 = f_ (one rank): an unrolled group of consecutive f_ touches consecutive
+         * elements of every unit-stride view. */
+        bool unit = false;
     };
     /* Grid-stride over [0, tot).  With `parts`, the range is cut into min(8, bh_nblk) contiguous
      * parts, part x walked by the workgroups b with b mod 8 = x (the workgroups one XCD runs), so a
@@ -850,6 +853,46 @@ void Writer::grid_stride(const std::string &tot_in, int unroll, const Element &e
         body << "  const int64_t st_ = (int64_t)bh_nblk * BH_WG;\n";
         body << "  int64_t g_ = (int64_t)bh_bid * BH_WG + threadIdx.x;\n";
     }
+    if (unroll > 1 && e.unit && !parts) {
+        // Groups of `unroll` consecutive elements: group q_ is [q_ * unroll, q_ * unroll + unroll), the
+        // groups are walked grid-stride, so a lane's loads of one unit-stride view are adjacent (the
+        // compiler merges them into one wide load, e.g. 16 bytes for 4 floats) and a wave's lanes read
+        // consecutive 4 * unroll-byte pieces.  The last tot % unroll elements go to the first threads.
+        const std::vector<const bh_view *> reads =
+            e.body_loop != nullptr ? preloadable(*e.body_loop) : std::vector<const bh_view *>();
+        const std::string U = std::to_string(unroll);
+        body << "  const int64_t g0_ = g_, ng_ = " << tot << " / " << U << ";\n";
+        body << "  for (; g_ < ng_; g_ += st_) {\n";
+        for (int u = 0; u < unroll; ++u)
+            for (size_t k = 0; k < reads.size(); ++k)
+                body << "    " << hip_type(reads[k]->base->dtype()) << " ld" << k << "_" << u << ";\n";
+        for (int u = 0; u < unroll; ++u) {
+            if (reads.empty()) break;
+            body << "    {\n      const int64_t f_ = g_ * " << U << " + " << u << ";\n";
+            e.decode(3);
+            for (size_t k = 0; k < reads.size(); ++k)
+                body << "      ld" << k << "_" << u << " = " << ref(*e.outer, *reads[k]) << ";\n";
+            body << "    }\n";
+        }
+        for (int u = 0; u < unroll; ++u) {
+            Scope pre;
+            pre.parent = e.outer;
+            for (size_t k = 0; k < reads.size(); ++k)
+                pre.preload[vkey(*reads[k])] = "ld" + std::to_string(k) + "_" + std::to_string(u);
+            body << "    {\n      const int64_t f_ = g_ * " << U << " + " << u << ";\n";
+            e.decode(3);
+            e.compute(3, &pre);
+            body << "    }\n";
+        }
+        body << "  }\n";
+        body << "  g_ = ng_ * " << U << " + g0_;\n";
+        body << "  if (g_ < " << tot << ") {\n";
+        body << "    const int64_t f_ = g_;\n";
+        e.decode(2);
+        e.compute(2, e.outer);
+        body << "  }\n";
+        return;
+    }
     if (unroll > 1) {
         const std::vector<const bh_view *> reads =
             e.body_loop != nullptr ? preloadable(*e.body_loop) : std::vector<const bh_view *>();
@@ -1079,6 +1122,7 @@ void Writer::emit_map() {
     e.outer = &ksc;
     e.body_loop = flat ? chain[p - 1] : nullptr;
     if (p == 2) e.inner = "n" + std::to_string(lid.at(chain[1]));
+    e.unit = p == 1;
     e.decode = [&](int ind) {
         const std::string pad(2 * ind, ' ');
         if (p == 1) {
@@ -1296,6 +1340,7 @@ void Writer::emit_full_reduce() {
         Element e;
         e.outer = &ksc;
         e.body_loop = l0.isInnermost() ? &l0 : nullptr;
+        e.unit = true;
         e.decode = [&](int ind) { body << std::string(2 * ind, ' ') << "const int64_t i0 = f_;\n"; };
         e.compute = [&](int ind, const Scope *outer) {
             Scope sc;

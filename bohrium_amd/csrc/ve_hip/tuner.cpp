@@ -64,7 +64,7 @@ std::vector<LaunchKnobs> Tuner::candidates(Strategy s, const LaunchKnobs &base) 
     static const int grids[] = {1024, 2048, 4096, 8192};
     switch (s) {
         case Strategy::MAP:
-            for (int u : {1, 2})
+            for (int u : {1, 2, 4})
                 for (int g : grids) {
                     LaunchKnobs k = base;
                     k.max_grid = g;
@@ -73,7 +73,7 @@ std::vector<LaunchKnobs> Tuner::candidates(Strategy s, const LaunchKnobs &base) 
                 }
             break;
         case Strategy::FULL_REDUCE:
-            for (int u : {1, 2})
+            for (int u : {1, 2, 4, 8})
                 for (int g : grids) {
                     LaunchKnobs k = base;
                     k.reduce_max_grid = g;
