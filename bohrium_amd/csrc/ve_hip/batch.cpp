@@ -317,6 +317,9 @@ bool EngineHip::launchPersist(PersistPlan &pp, CachedFlush &cf, const std::vecto
         ++persist_fallbacks;
         return false;
     }
+    // The stop event goes right behind the launch: recorded after the host work below, it would time
+    // that work too whenever the device finishes first.
+    if (profile) bhhip_event_record(ev1);
     ++stats.launches;
     ++persist_launches;
     ++stats.launches_by_strategy["persistent"];
@@ -324,7 +327,6 @@ bool EngineHip::launchPersist(PersistPlan &pp, CachedFlush &cf, const std::vecto
     for (const PlanRef &r : pp.plans)
         for (int o : cf.kernels[r.kernel].plans[r.plan].written_ordinals) noteWrite(ord2base[o], buffer(ord2base[o]));
     if (profile) {
-        bhhip_event_record(ev1);
         pending_events.push_back({ev0, ev1, {"persistent", work, pp.grid, 0.0,
                                              "plans=" + std::to_string(pp.plans.size()) + " tasks=" +
                                                  std::to_string(pp.tasks.size() / 4) + " levels=" +
@@ -624,6 +626,7 @@ void EngineHip::launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh
                                   {reinterpret_cast<uint8_t *>(&f), reinterpret_cast<uint8_t *>(&f) + sizeof(f)}});
         ++stats.launches;
     }
+    if (profile) bhhip_event_record(ev1);  // right behind the launches (see launchPersist)
     ++batch_launches;
     batched_items += k;
     stats.launches_by_strategy[std::string("batch-") + strategy_name(p0.strategy)] += 1;
@@ -631,7 +634,6 @@ void EngineHip::launchBatch(BatchGroup &g, CachedFlush &cf, const std::vector<bh
         for (int o : cf.kernels[r.kernel].plans[r.plan].written_ordinals) noteWrite(ord2base[o], buffer(ord2base[o]));
     }
     if (profile) {
-        bhhip_event_record(ev1);
         pending_events.push_back({ev0, ev1, {std::string("batch-") + strategy_name(p0.strategy), work, blocks, 0.0,
                                              "items=" + std::to_string(k), 0, ""}});
     }

@@ -425,10 +425,10 @@ void EngineHip::launchPlan(KernelPlan &plan, const std::vector<bh_base *> &ord2b
         ++stats.launches;
         if (recording != nullptr) recording->push_back({plan.fin_function, 1, plan.block, {buf, buf + n}});
     }
+    if (profile) bhhip_event_record(ev1);  // right behind the launches, before the host work below
     ++stats.launches_by_strategy[strategy_name(plan.strategy)];
     for (int o : plan.written_ordinals) noteWrite(ord2base[o], buffer(ord2base[o]));
     if (profile) {
-        bhhip_event_record(ev1);
         char hex[32];
         std::snprintf(hex, sizeof(hex), "%016llx", static_cast<unsigned long long>(plan.source_hash));
         pending_events.push_back({ev0, ev1, {strategy_name(plan.strategy), plan.work_items, plan.grid, 0.0, hex,

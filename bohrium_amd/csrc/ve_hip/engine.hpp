@@ -211,7 +211,7 @@ private:
         d.dev_valid = true;
         d.host_valid = false;
         ++d.writes;
-        flush_written.insert(base);
+        if (recording != nullptr) flush_written.insert(base);  // only a recorded flush's effects are kept
     }
     /* Bases released during the current flush; a BH_FREE that no kernel released is caught with it. */
     std::unordered_set<const bh_base *> released_now;
