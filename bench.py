@@ -32,7 +32,9 @@ TIME_STEPS = 10
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
+    # 30 timed runs (about 50 ms): the host queues a run in a third of its device time, so after the
+    # first run's start-up (about 0.7 ms) the device stays busy; fewer runs weigh the start-up in more.
+    p.add_argument("--steps", type=int, default=30)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--n", type=int, default=20_000_000)
     p.add_argument("--no-cpu", action="store_true")
