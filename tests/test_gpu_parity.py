@@ -356,9 +356,10 @@ def test_jacobi_full_grid_20_sweeps():
     assert_parity(got[0], want[0], np.float32, rtol=1e-5, atol=1e-3)
 
 
-@pytest.mark.parametrize("unroll", [1, 2, 3, 4])
+@pytest.mark.parametrize("unroll", [1, 2, 3, 4, 8])
 def test_unrolled_grid_stride_parity(unroll):
-    """Unrolled map and reduce skeletons (sizes that leave a remainder) against the oracle."""
+    """Unrolled map and reduce skeletons (sizes that leave a remainder) against the oracle; one-rank walks
+    take groups of `unroll` consecutive elements (merged wide loads), 2-D ones grid-strided groups."""
     old = bh.message("codegen:")
     bh.message("codegen: map_unroll=%d reduce_unroll=%d max_grid=64" % (unroll, unroll))
     try:
