@@ -204,6 +204,20 @@ def new(dtype, nelem):
     return h
 
 
+def direct_ready():
+    """True when instructions can go straight into the extension: no trace, no recording."""
+    return _fast is not None and _trace is None and _initialised[0] and not _record_only[0]
+
+
+def op_direct(opcode, operands, ctype, cbytes):
+    """Queue one instruction straight into the extension (callers check direct_ready()): `operands`
+    as for op(), the constant as its bridge type code (-1: none) and its bytes."""
+    try:
+        _fast.op(opcode, operands, ctype, cbytes)
+    except RuntimeError as e:
+        raise BohriumError(str(e))
+
+
 def make_view(handle, start, shape, stride):
     v = BhcView()
     v.base = handle

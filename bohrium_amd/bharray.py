@@ -82,6 +82,22 @@ class BhArray:
         # the repeats of do_while (views indexed by an Iterator).
         self.slides = tuple(slides)
 
+    @classmethod
+    def _fresh(cls, shape, dtype):
+        """A new contiguous array of `shape` (a tuple of ints) and `dtype` (a numpy dtype), both taken
+        from an existing array, so without the constructor's conversions."""
+        self = object.__new__(cls)
+        self.shape = shape
+        self.dtype = dtype
+        n = 1
+        for d in shape:
+            n *= d
+        self.base = BhBase(dtype, n)
+        self.offset = 0
+        self.strides = _contiguous_strides(shape)
+        self.slides = ()
+        return self
+
     # ------------------------------------------------------------ basics
     @property
     def ndim(self):
@@ -395,12 +411,20 @@ def ops_identity(src, dst):
 
 
 _CONSTS = {}
+_CONST_BYTES = {}  # constant key -> (bridge type code, bytes), for ufunc._call_direct
+# ufunc calls take ufunc._call_direct when it applies (a switch for tests that compare both paths)
+_DIRECT = [True]
+
+
+def _const_key(value, dtype):
+    # 0.0 == -0.0 as dictionary keys: zeros are told apart by their text
+    return (type(value), value if value or type(value) is not float else repr(value), dtype)
 
 
 def _const_scalar(value, dtype):
     """A constant in the instruction's type, converted the way the reference's bridges do (C cast)."""
     if type(value) in (int, float, bool):
-        key = (type(value), value, dtype)
+        key = _const_key(value, dtype)
         c = _CONSTS.get(key)
         if c is None:
             c = np.asarray(value).astype(np.dtype(dtype))[()]
@@ -588,6 +612,10 @@ class Ufunc:
         return res[:self.nin], res[self.nin]
 
     def __call__(self, *args, out=None):
+        if out is None and _DIRECT[0] and api.direct_ready():
+            res = self._call_direct(args)
+            if res is not None:
+                return res
         if len(args) != self.nin:
             raise TypeError("%s takes %d inputs" % (self.name, self.nin))
         in_types, out_type = self._loop_dtypes(args)
@@ -622,6 +650,50 @@ class Ufunc:
         if out is not None and res is not out:
             ops_identity(res, out)
             return out
+        return res
+
+    def _call_direct(self, args):
+        """The common calls without the general machinery: array operands of one shape, or an array and
+        a Python int or float, no slides, loop types equal to the arrays' own (no cast); the instruction
+        goes straight into the extension.  None for every other call."""
+        op = self.opcode
+        if op is None or callable(op) or len(args) != self.nin:
+            return None
+        a = args[0]
+        if type(a) is not BhArray or a.slides:
+            return None
+        if self.nin == 1:
+            hit = self._cache.get((a.dtype,)) or self._loop_dtypes(args)
+            if hit[0][0] != a.dtype:
+                return None
+            res = BhArray._fresh(a.shape, hit[1])
+            api.op_direct(op, [res.view_desc(), a.view_desc()], -1, None)
+            return res
+        b = args[1]
+        tb = type(b)
+        if tb is BhArray:
+            if b.slides or b.shape != a.shape:
+                return None
+            hit = self._cache.get((a.dtype, b.dtype)) or self._loop_dtypes(args)
+            if hit[0][0] != a.dtype or hit[0][1] != b.dtype:
+                return None
+            res = BhArray._fresh(a.shape, hit[1])
+            api.op_direct(op, [res.view_desc(), a.view_desc(), b.view_desc()], -1, None)
+            return res
+        if tb is not float and tb is not int:
+            return None
+        hit = self._cache.get((a.dtype, tb)) or self._loop_dtypes(args)
+        if hit[0][0] != a.dtype:
+            return None
+        key = _const_key(b, hit[0][1])
+        cb = _CONST_BYTES.get(key)
+        if cb is None:
+            c = _const_scalar(b, hit[0][1])
+            cb = (api.BH_TYPES[c.dtype], c.tobytes())
+            if len(_CONST_BYTES) < 65536:
+                _CONST_BYTES[key] = cb
+        res = BhArray._fresh(a.shape, hit[1])
+        api.op_direct(op, [res.view_desc(), a.view_desc(), None], cb[0], cb[1])
         return res
 
     def _emit(self, res, operands, const):
