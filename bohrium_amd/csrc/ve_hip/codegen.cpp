@@ -807,9 +807,11 @@ std::vector<const bh_view *> Writer::preloadable(const LoopB &l) {
 }
 
 // Grid-stride loop over the flat index f_ in [0, tot).  With unroll > 1, full groups of
-// `unroll` elements (strided by the grid) first load every read-only input of all of them
-// into registers and then compute them, so a wave has all of its loads in flight at once;
-// the remaining elements run one at a time.
+// `unroll` elements first load every read-only input of all of them into registers and then
+// compute them, so a wave has all of its loads in flight at once; the remaining elements run one
+// at a time.  A group is `unroll` consecutive elements on a one-rank walk without XCD parts (a
+// unit-stride view's loads merge into wide loads), else `unroll` elements strided by the grid.
+
 // Full reductions with at most this many instructions in their loop count as light (half the grid).
 constexpr size_t kLightReduceInstrs = 16;
 
