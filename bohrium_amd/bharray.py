@@ -256,6 +256,10 @@ class BhArray:
         return out
 
     def astype(self, dtype):
+        if not self.slides:
+            out = BhArray._fresh(self.shape, dtype if type(dtype) is _DTYPE else np.dtype(dtype))
+            _op(ops.IDENTITY, [out.view_desc(), self.view_desc()])
+            return out
         out = BhArray(self.shape, dtype)
         ops_identity(self, out)
         return out
@@ -412,8 +416,29 @@ def ops_identity(src, dst):
 
 _CONSTS = {}
 _CONST_BYTES = {}  # constant key -> (bridge type code, bytes), for ufunc._call_direct
-# ufunc calls take ufunc._call_direct when it applies (a switch for tests that compare both paths)
+# ufunc calls take ufunc._call_direct when it applies, other calls _op's direct form (a switch for
+# tests that compare both paths)
 _DIRECT = [True]
+_AXES = {}  # axis -> its constant (int64 scalar)
+
+
+def _op(opcode, descs, const=None, slides=None):
+    """api.op, straight into the bridge extension when nothing needs the general path (no slides, no
+    trace or recording)."""
+    if _DIRECT[0] and not slides and api.direct_ready():
+        if const is None:
+            api.op_direct(opcode, descs, -1, None)
+        else:
+            api.op_direct(opcode, descs, api.BH_TYPES[const.dtype], const.tobytes())
+        return
+    api.op(opcode, descs, const=const, slides=slides)
+
+
+def _axis_const(axis):
+    c = _AXES.get(axis)
+    if c is None:
+        c = _AXES[axis] = np.int64(axis)
+    return c
 
 
 def _const_key(value, dtype):
@@ -529,9 +554,8 @@ def take(a, indices):
     idx = indices if isinstance(indices, BhArray) else array(indices, np.int64)
     if idx.dtype != np.int64:
         idx = idx.astype(np.int64)
-    out = BhArray(idx.shape, a.dtype)
-    flat = BhArray((a.size,), base=a.base, offset=a.offset)
-    api.op(ops.GATHER, [out.view_desc(), flat.view_desc(), idx.view_desc()])
+    out = BhArray._fresh(idx.shape, a.dtype)
+    _op(ops.GATHER, [out.view_desc(), (a.base.handle, a.offset, (a.size,), (1,)), idx.view_desc()])
     return out
 
 
@@ -727,7 +751,7 @@ class Ufunc:
             return self.reduce(r, axis[:-1], out)
         ax = axis[0]
         shape = tuple(s for i, s in enumerate(ary.shape) if i != ax)
-        tmp = BhArray(shape, ary.dtype)
+        tmp = BhArray._fresh(shape, ary.dtype)
         if ary.shape[ax] == 0:
             ident = {"add": 0, "multiply": 1, "logical_or": False, "logical_and": True}.get(self.name)
             if ident is None:
@@ -736,8 +760,8 @@ class Ufunc:
         elif ary.shape[ax] == 1:
             ops_identity(ary[(slice(None),) * ax + (0,)], tmp)
         else:
-            api.op(self.reduce_opcode, [tmp.view_desc(), ary.view_desc(), None], const=np.int64(ax),
-                   slides=_slides_of([tmp, ary]))
+            _op(self.reduce_opcode, [tmp.view_desc(), ary.view_desc(), None], const=_axis_const(ax),
+                slides=_slides_of([tmp, ary]))
         if out is not None:
             assign(tmp, out)
             return out
@@ -755,9 +779,9 @@ class Ufunc:
             dt = np.dtype(np.uint64)
         if dt != ary.dtype:
             ary = ary.astype(dt)
-        res = BhArray(ary.shape, dt)
+        res = BhArray._fresh(ary.shape, dt)
         if ary.size:
-            api.op(self.accumulate_opcode, [res.view_desc(), ary.view_desc(), None], const=np.int64(axis))
+            _op(self.accumulate_opcode, [res.view_desc(), ary.view_desc(), None], const=_axis_const(axis))
         if out is not None:
             assign(res, out)
             return out

@@ -39,6 +39,8 @@ outs = [
     i * 3 + 7, i // 2, i %% 5,                 # int64 with int constants
     d * -0.0, d + 0.0, d * 1e300,              # signed zero and large float constants
     bh.ufunc("absolute")(-d), x * 2,           # float32 array with an int constant
+    x.sum(), d.max(), (x > 0).astype(np.int64).cumsum(), i.astype(np.float32),
+    bh.take(x, i // 2), bh.ones((30, 40)).sum(axis=1),
 ]
 bh.flush()
 print("DIRECT_CALLS=%%d" %% taken[0])
