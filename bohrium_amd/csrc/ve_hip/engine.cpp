@@ -559,6 +559,7 @@ size_t EngineHip::packArgs(const KernelPlan &plan, const std::vector<bh_base *> 
 std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std::vector<bh_base *> &ord2base) const {
     std::vector<int64_t> words;  // the key as 64-bit words (one string made at the end)
     words.reserve(instrs.size() * 24);
+    words.push_back(0);  // the hash of the words after it (FlushKeyHash reads it instead of the whole key)
     FlatPtrMap base2ord(instrs.size());
     auto put = [&](int64_t v) { words.push_back(v); };
     // Equal constants share a kernel parameter: the key holds, for each constant, the position of the
@@ -612,6 +613,9 @@ std::string EngineHip::flushKey(const std::vector<bh_instruction *> &instrs, std
             }
         }
     }
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 1; i < words.size(); ++i) h = (h ^ static_cast<uint64_t>(words[i])) * 0x100000001b3ull;
+    words[0] = static_cast<int64_t>(h ^ (h >> 29));
     return std::string(reinterpret_cast<const char *>(words.data()), words.size() * sizeof(int64_t));
 }
 
@@ -729,6 +733,7 @@ void EngineHip::handleExecution(BhIR *bhir) {
     std::vector<bh_base *> ord2base;
     const std::string key = flushKey(instr_list, ord2base);
     stats.time_key += seconds_since(t);
+    t = std::chrono::steady_clock::now();  // time_fuse: the lookup, and on a miss the fusion
     auto hit = flush_cache.find(key);
     static const bool debug = std::getenv("BH_HIP_DEBUG") != nullptr;
     if (hit == flush_cache.end()) {

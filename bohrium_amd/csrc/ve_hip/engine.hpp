@@ -11,6 +11,7 @@
  */
 #pragma once
 
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -340,7 +341,17 @@ private:
     void compilePlan(KernelPlan &plan);
     void dumpLaunch(const char *dir, const KernelPlan &plan, const std::vector<bh_base *> &ord2base,
                     const std::vector<bh_instruction *> &instrs);
-    std::unordered_map<std::string, CachedFlush> flush_cache;
+    /* Flush keys start with a 64-bit hash of the rest (flushKey): hashing reads those 8 bytes, not the
+     * whole key (megabytes for flushes of tens of thousands of instructions); equality still compares
+     * the whole key. */
+    struct FlushKeyHash {
+        size_t operator()(const std::string &k) const noexcept {
+            uint64_t h = 0;
+            if (k.size() >= sizeof h) std::memcpy(&h, k.data(), sizeof h);
+            return static_cast<size_t>(h);
+        }
+    };
+    std::unordered_map<std::string, CachedFlush, FlushKeyHash> flush_cache;
     uint64_t flush_cache_hits = 0, flush_cache_misses = 0;
     uint64_t missed_frees = 0;
     uint64_t releases = 0;  // bases released (BH_FREE handled) since the engine started
