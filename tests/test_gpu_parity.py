@@ -240,7 +240,7 @@ def test_mc_pi_1e6_bit_exact():
 
 
 def test_mc_pi_full_config_bit_exact():
-    """BASELINE config: N = 1e7 each, seed 42; the count must be bit-exact."""
+    """The benchmark size: N = 1e7 each, seed 42; the count must be bit-exact."""
     got, want = run_both(lambda: workloads.mc_pi(10 ** 7))
     assert int(got[0]) == int(want[0])
     assert abs(4.0 * int(got[0]) / 1e7 - np.pi) < 2e-3
@@ -315,7 +315,7 @@ def test_sliding_views_known_answers():
 
 
 def test_black_scholes_repeat_full_size_sums():
-    """BASELINE config size (N = 2e7, 10 steps): engine vs oracle on the same bytecode."""
+    """The benchmark size (N = 2e7, 10 steps): engine vs oracle on the same bytecode."""
     def prog():
         S, K, T = workloads.black_scholes_inputs(20_000_000)
         return workloads.black_scholes_repeat(S, K, T, steps=10)
