@@ -353,8 +353,8 @@ def main():
         print(json.dumps(base))
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (see DESIGN.md for the "
-              "sharding design)" % args.gpus, file=sys.stderr)
+        print("bench.py: the engine runs on one GPU; --gpus %d is not supported (a sharded form would give each "
+              "rank a contiguous slice of every array; it is designed, not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import bohrium_amd as bh
     from bohrium_amd import workloads, _bh_api as api
