@@ -82,11 +82,10 @@ void *DevicePool::alloc(uint64_t bytes) {
     bytes = std::max<uint64_t>(256, (bytes + 255) / 256 * 256);
     // Last released first: a loop that frees and re-creates a temporary gets the same buffer
     // every iteration, so its launches (and a graph of them) stay identical.
-    auto range = _free.equal_range(bytes);
-    if (range.first != range.second) {
-        auto it = std::prev(range.second);
-        void *p = it->second;
-        _free.erase(it);
+    auto it = _free.find(bytes);
+    if (it != _free.end() && !it->second.empty()) {
+        void *p = it->second.back();
+        it->second.pop_back();
         _cached -= bytes;
         ++hits;
         return p;
@@ -105,25 +104,35 @@ void *DevicePool::alloc(uint64_t bytes) {
 void DevicePool::release(void *ptr, uint64_t bytes) {
     if (ptr == nullptr) return;
     bytes = std::max<uint64_t>(256, (bytes + 255) / 256 * 256);
-    _free.emplace(bytes, ptr);
+    _free[bytes].push_back(ptr);
     _cached += bytes;
     if (_cached > limit) trim(limit / 2);
 }
 
 void DevicePool::trim(uint64_t keep) {
-    while (_cached > keep && !_free.empty()) {
-        auto it = std::prev(_free.end());  // largest first
-        bhhip_synchronize();
-        bhhip_free(it->second);
+    bool synced = false;
+    while (_cached > keep) {
+        // the largest size first, the last released buffer of it first
+        auto largest = _free.end();
+        for (auto it = _free.begin(); it != _free.end(); ++it)
+            if (!it->second.empty() && (largest == _free.end() || it->first > largest->first)) largest = it;
+        if (largest == _free.end()) break;
+        if (!synced) {
+            bhhip_synchronize();
+            synced = true;
+        }
+        bhhip_free(largest->second.back());
+        largest->second.pop_back();
         ++frees;
-        _cached -= it->first;
-        allocated -= it->first;
-        _free.erase(it);
+        _cached -= largest->first;
+        allocated -= largest->first;
+        if (largest->second.empty()) _free.erase(largest);
     }
 }
 
 DevicePool::~DevicePool() {
-    for (auto &e : _free) bhhip_free(e.second);
+    for (auto &e : _free)
+        for (void *p : e.second) bhhip_free(p);
 }
 
 // ------------------------------------------------------------------ engine

@@ -46,7 +46,9 @@ struct DeviceBuffer {
 
 /* Size-keyed reuse of device allocations (hipMalloc is far slower than a lookup). */
 class DevicePool {
-    std::multimap<uint64_t, void *> _free;
+    /* Released buffers by exact size, the last released at the back (constant time per release and
+     * reuse: a flush of thousands of temporaries releases and takes thousands of buffers). */
+    std::unordered_map<uint64_t, std::vector<void *> > _free;
     uint64_t _cached = 0;
 
 public:
