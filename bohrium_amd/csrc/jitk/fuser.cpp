@@ -112,6 +112,19 @@ std::vector<InstrPtr> simplify_instr_list(const std::vector<bh_instruction *> &i
     return ret;
 }
 
+namespace {
+// Whether `i` reads or writes an array that an instruction of `g` reads or writes.
+bool shares_base(const std::vector<InstrPtr> &g, const InstrPtr &i) {
+    for (const bh_view &v : i->operand) {
+        if (v.isConstant()) continue;
+        for (const InstrPtr &j : g)
+            for (const bh_view &w : j->operand)
+                if (!w.isConstant() && w.base == v.base) return true;
+    }
+    return false;
+}
+} // namespace
+
 std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_list, bool accumulate_alone,
                                    bool one_sweep_axis) {
     const std::vector<InstrPtr> list = simplify_instr_list(instr_list);
@@ -162,12 +175,17 @@ std::vector<Block> pre_fuser_lossy(const std::vector<bh_instruction *> &instr_li
                         ++it;
                         for (; it != list.end(); ++it) {
                             if (bh_opcode_is_sweep((*it)->opcode) || !fully_fusible(n, *it)) break;
+                            if (!bh_opcode_is_system((*it)->opcode) && !shares_base(n, *it)) break;  // as below
                             n.push_back(*it);
                         }
                     }
                 }
                 break;
             }
+            // A group with an accumulate (a GPU scan) takes no instruction that shares no array with
+            // it: that work would only wait on the scan's tiles.
+            if (accumulate_alone && has_acc && !acc && !bh_opcode_is_system((*it)->opcode) && !shares_base(g, *it))
+                break;
             if (fully_fusible(g, *it)) {
                 g.push_back(*it);
                 has_acc |= acc;
